@@ -1,0 +1,187 @@
+/*
+ * lsx.h — C-ABI of the MI355X (gfx950) dense-LU / SimpleGMRES library (liblsx.so).
+ *
+ * Conventions (those of the reference's Julia arrays and LAPACK):
+ *   - matrices are column-major with an explicit leading dimension;
+ *   - pivots are 1-based int64 LAPACK swap sequences (ipiv[k] = row interchanged with row k);
+ *   - every pointer argument may point to host memory or to device memory of the context's GPU
+ *     (the library detects which), except where a function says "device";
+ *   - every function returns int32: 0 = success, > 0 = LAPACK-style info (1-based index of the
+ *     first exactly-zero pivot; the factorization still ran to completion), < 0 = error
+ *     (one of lsx_status below); lsx_last_error() then returns a message.  No C++ exception
+ *     crosses this boundary.
+ *   - one handle = one HIP stream; handles are independent and may be used from different host
+ *     threads; a single handle must not be used by two threads at once.
+ *
+ * Reference citations are paths in LinearSolve.jl v5.12.0.
+ */
+#ifndef LSX_H
+#define LSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    LSX_OK = 0,
+    LSX_EINVAL = -1,   /* bad argument */
+    LSX_EHIP = -2,     /* HIP runtime error */
+    LSX_ENOMEM = -3,   /* device allocation failed or over the HBM budget */
+    LSX_ESTATE = -4,   /* call out of order (e.g. solve before a successful factor) */
+    LSX_ETIMEOUT = -5, /* a device-side wait exceeded its bound (results invalid) */
+    LSX_EAGAIN = -6    /* result not ready yet (serving poll) */
+} lsx_status;
+
+typedef enum { LSX_F64 = 0, LSX_F32 = 1, LSX_BF16 = 2 } lsx_dtype;
+
+/* SciMLBase.ReturnCode values used by the hot paths (simplegmres.jl:603-606,
+ * factorization.jl:716-722). */
+typedef enum {
+    LSX_RET_DEFAULT = 0,
+    LSX_RET_SUCCESS = 1,
+    LSX_RET_MAXITERS = 2,
+    LSX_RET_INFEASIBLE = 3,
+    LSX_RET_FAILURE = 4
+} lsx_retcode;
+
+typedef struct lsx_ctx lsx_ctx;
+typedef struct lsx_lu lsx_lu;
+typedef struct lsx_batch lsx_batch;
+typedef struct lsx_csr lsx_csr;
+typedef struct lsx_serve lsx_serve;
+
+/* ---------------------------------------------------------------- context and memory ----- */
+/* A context binds one GPU.  Stands in for the device side of LinearCache.cacheval
+ * (src/common.jl:281-306). */
+int lsx_ctx_create(int device, lsx_ctx** out);
+int lsx_ctx_destroy(lsx_ctx* ctx);
+const char* lsx_last_error(const lsx_ctx* ctx);
+int lsx_version(void);
+/* Device memory helpers for hosts without a GPU array package (device pointers). */
+int lsx_malloc(lsx_ctx* ctx, size_t bytes, void** dptr);
+int lsx_free(lsx_ctx* ctx, void* dptr);
+int lsx_memcpy(lsx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int lsx_synchronize(lsx_ctx* ctx);
+
+/* ---------------------------------------------------------------- dense LU --------------- */
+/* Handle of one n x n factorization kept resident on the device: replaces the LU object that
+ * LUFactorization stores in cache.cacheval (src/factorization.jl:714, init_cacheval :840-849).
+ * dtype LSX_F64 = plain partial-pivot LU (LAPACK dgetrf semantics); LSX_F32 / LSX_BF16 = the
+ * mixed-precision factorization used by lsx_lu_refine. */
+int lsx_lu_create(lsx_ctx* ctx, int64_t n, int dtype, lsx_lu** out);
+int lsx_lu_destroy(lsx_lu* lu);
+
+/* getrf: copies A (host or device, lda >= n) into the handle and factors it.
+ * Replaces lu(A, check=false) / LAPACK.getrf!(factors, ipiv) at src/factorization.jl:689-697
+ * (ccall path src/openblas.jl:144-151).  Returns info (> 0: first zero pivot, as LAPACK). */
+int lsx_lu_factor(lsx_lu* lu, const double* A, int64_t lda);
+
+/* Copies the resident factors (L unit-lower below the diagonal, U on and above) and the
+ * 1-based pivots out; either pointer may be NULL.  ipiv has n entries. */
+int lsx_lu_get_factors(lsx_lu* lu, double* F, int64_t ldf, int64_t* ipiv);
+
+/* getrs: X = A \ B for nrhs right-hand sides with the resident factors.  B and X may alias.
+ * Replaces _smart_lu_ldiv!/ldiv!(F, b) -> LAPACK dgetrs at src/factorization.jl:601-611,727-728
+ * (ccall path src/openblas.jl:267-275). */
+int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
+
+/* Statistics of the last lsx_lu_refine call. */
+typedef struct {
+    int32_t mode_used;      /* 0 = iterative refinement, 1 = GMRES-IR */
+    int32_t iterations;     /* refinement steps (IR) or total GMRES iterations (GMRES-IR) */
+    int32_t outer;          /* refinement sweeps */
+    int32_t converged;      /* 1 if the backward error reached tol */
+    double backward_error;  /* max over columns of ||b-Ax||_inf/(||A||_inf ||x||_inf+||b||_inf) */
+} lsx_refine_stats;
+
+/* Mixed-precision solve of the factored lu (LSX_F32/LSX_BF16, or LSX_F64) with FP64 refinement
+ * to a normwise backward error <= tol.  mode 0 = IR, 1 = GMRES-IR, 2 = IR then GMRES-IR if IR
+ * stalls.  A (host or device) is the original FP64 matrix when it is not already held by the
+ * handle (pass NULL after lsx_lu_factor: the handle keeps its FP64 copy).  No reference
+ * counterpart does refinement: the API shape follows OpenBLAS32MixedLUFactorization
+ * (src/openblas.jl:461-543), whose FP32 result is refined here to FP64 accuracy. */
+int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,
+                  double tol, int32_t max_iters, int32_t mode, lsx_refine_stats* stats);
+
+/* ---------------------------------------------------------------- batched dense LU ------- */
+/* count independent systems of sizes n[i] (1 <= n[i] <= 512), packed back to back:
+ * A_i column-major at offset sum_{j<i} n_j^2, b_i / x_i at offset sum_{j<i} n_j.
+ * Replaces the per-block lu!/ldiv! loop of BlockDiagonalFactorization
+ * (ext/LinearSolveBlockDiagonalsExt.jl:101-114,123-185). */
+int lsx_batch_create(lsx_ctx* ctx, int64_t count, const int64_t* n, int dtype, lsx_batch** out);
+int lsx_batch_destroy(lsx_batch* batch);
+/* Factor and solve every system in one persistent launch.  info[i] (nullable, host or device)
+ * receives each system's LAPACK info; ipiv (nullable) receives the pivots packed like b.
+ * Returns the number of systems with info > 0 (0 = all factored). */
+int lsx_batch_factor_solve(lsx_batch* batch, const double* A, const double* b, double* x,
+                           int64_t* info, int64_t* ipiv);
+
+/* ---------------------------------------------------------------- sparse / SimpleGMRES --- */
+/* CSR matrix resident on the device (int32 row pointers and column indices, FP64 values). */
+int lsx_csr_create(lsx_ctx* ctx, int64_t N, int64_t nnz, const int32_t* rowptr,
+                   const int32_t* colind, const double* vals, lsx_csr** out);
+int lsx_csr_destroy(lsx_csr* A);
+/* y = A x (device or host vectors). */
+int lsx_csr_spmv(lsx_csr* A, const double* x, double* y);
+
+typedef struct {
+    int32_t retcode;   /* lsx_retcode */
+    int32_t pad;
+    int64_t iters;     /* LinearSolution.iters */
+    double resid;      /* LinearSolution.resid: the preconditioned residual estimate rNorm */
+} lsx_gmres_stats;
+
+/* SimpleGMRES (src/simplegmres.jl:375-616, scalar layout): restarted GMRES with modified
+ * Gram-Schmidt Arnoldi and symmetric Givens rotations.  pl_diag / pr_diag: diagonal left/right
+ * preconditioners (ldiv! divides elementwise), NULL = identity.  x receives the solution
+ * (warm_start = false: the iteration starts from zero).  The Krylov workspace stays resident on
+ * the handle between calls. */
+int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int32_t restart,
+                    double abstol, double reltol, int64_t maxiters, const double* pl_diag,
+                    const double* pr_diag, lsx_gmres_stats* stats);
+
+/* ---------------------------------------------------------------- serving ---------------- */
+typedef struct {
+    int32_t kind;          /* 0 = dense LU solve, 1 = SimpleGMRES */
+    int32_t dtype;         /* dense: LSX_F64 */
+    int64_t n;             /* dense order or sparse N */
+    const double* A;       /* dense: n x n column-major (lda = n) */
+    const double* b;       /* length n */
+    int64_t nnz;           /* sparse */
+    const int32_t* rowptr; /* sparse CSR */
+    const int32_t* colind;
+    const double* vals;
+    const double* pl_diag; /* sparse: Jacobi diagonal or NULL */
+    int32_t memory, restart;
+    double abstol, reltol;
+    int64_t maxiters;
+} lsx_request;
+
+typedef struct {
+    int32_t status;     /* 0 done, LSX_EAGAIN pending, < 0 error */
+    int32_t retcode;    /* lsx_retcode */
+    int64_t info;       /* dense: LAPACK info */
+    int64_t iters;      /* sparse: GMRES iterations */
+    double resid;       /* sparse: rNorm */
+    double t_submit, t_start, t_done; /* seconds on the serving clock */
+} lsx_result;
+
+/* A continuous-batching request loop: requests are admitted while their estimated HBM footprint
+ * fits hbm_budget bytes, small dense systems are grouped into batched launches, large ones and
+ * sparse solves run on their own streams.  The request's host buffers must stay valid until the
+ * result is polled. */
+int lsx_serve_create(lsx_ctx* ctx, size_t hbm_budget, int32_t nstreams, lsx_serve** out);
+int lsx_serve_destroy(lsx_serve* s);
+int lsx_serve_submit(lsx_serve* s, const lsx_request* req, double* x_out, uint64_t* ticket);
+/* Returns LSX_OK with *res filled when the ticket is done, LSX_EAGAIN while pending. */
+int lsx_serve_poll(lsx_serve* s, uint64_t ticket, lsx_result* res);
+int lsx_serve_drain(lsx_serve* s);
+double lsx_serve_clock(lsx_serve* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

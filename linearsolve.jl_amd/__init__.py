@@ -1,0 +1,605 @@
+"""Host-side mirror of LinearSolve.jl's LinearProblem / init / solve! / LinearCache interface for
+the gfx950 hot paths.
+
+Reference interface (paths in LinearSolve.jl v5.12.0):
+  LinearProblem + init + solve!          src/common.jl:700-712, 758-931, 988-1017
+  LinearCache fields and setproperty!    src/common.jl:281-306, 313-360 (A set -> isfresh; b set)
+  LUFactorization.solve!                 src/factorization.jl:643-736
+  SimpleGMRES.solve!                     src/simplegmres.jl:375-616
+  BlockDiagonalFactorization             ext/LinearSolveBlockDiagonalsExt.jl:48-185
+  ReturnCode / LinearSolution            SciMLBase (build_linear_solution, src/common.jl:748-753)
+
+`solve_(cache)` is `solve!(cache)` (Python has no `!`).  All arithmetic runs in liblsx.so on the
+GPU; this module only moves arrays, keeps handles in `cache.cacheval` and maps return codes.
+"""
+import ctypes
+import enum
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import LSX_BF16, LSX_F32, LSX_F64, LSXError, Context
+
+__all__ = [
+    "ReturnCode", "LinearProblem", "LinearCache", "LinearSolution", "init", "solve", "solve_",
+    "MI355XLUFactorization", "MI355XMixedLUFactorization", "MI355XBatchedLUFactorization",
+    "MI355XSimpleGMRES", "BlockDiagonal", "CSRMatrix", "LSXError", "Context", "Server",
+    "default_tol",
+]
+
+
+class ReturnCode(enum.IntEnum):
+    """Subset of SciMLBase.ReturnCode used by the hot paths (values match lsx_retcode)."""
+    Default = 0
+    Success = 1
+    MaxIters = 2
+    Infeasible = 3
+    Failure = 4
+
+
+def successful_retcode(rc):
+    return rc == ReturnCode.Success
+
+
+def default_tol(dtype=np.float64):
+    """src/common.jl:411: sqrt(eps(T))."""
+    return math.sqrt(np.finfo(dtype).eps)
+
+
+# -------------------------------------------------------------------------------------------------
+# array plumbing (numpy host arrays or torch tensors on the context's GPU)
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _colmajor(x):
+    """Returns (keepalive, pointer) of a column-major float64 copy/view of a vector or matrix."""
+    if _is_torch(x):
+        import torch
+        t = x.to(torch.float64)
+        if t.dim() == 2:
+            t = t.t().contiguous()  # row-major storage of A^T == column-major storage of A
+        else:
+            t = t.contiguous()
+        return t, t.data_ptr()
+    a = np.asfortranarray(np.asarray(x, dtype=np.float64))
+    return a, a.ctypes.data
+
+
+def _empty_like_rhs(b):
+    if _is_torch(b):
+        import torch
+        if b.dim() == 2:
+            return torch.empty((b.shape[1], b.shape[0]), dtype=torch.float64, device=b.device).t()
+        return torch.empty(b.shape, dtype=torch.float64, device=b.device)
+    return np.empty(b.shape, dtype=np.float64, order="F")
+
+
+def _ptr_out(u):
+    if _is_torch(u):
+        return u.data_ptr()
+    return u.ctypes.data
+
+
+def _ncols(b):
+    return 1 if len(b.shape) == 1 else int(b.shape[1])
+
+
+def _copy(x):
+    if _is_torch(x):
+        return x.clone()
+    return np.array(x, copy=True)
+
+
+# -------------------------------------------------------------------------------------------------
+# problem, cache, solution
+
+class LinearProblem:
+    """A x = b (SciMLBase.LinearProblem)."""
+
+    def __init__(self, A, b, u0=None):
+        self.A = A
+        self.b = b
+        self.u0 = u0
+
+
+class LinearSolution:
+    """Fields of SciMLBase.LinearSolution used by the reference (src/common.jl:748-753)."""
+
+    def __init__(self, u, resid=None, retcode=ReturnCode.Success, iters=0, cache=None, stats=None):
+        self.u = u
+        self.resid = resid
+        self.retcode = retcode
+        self.iters = iters
+        self.cache = cache
+        self.stats = stats
+
+    @property
+    def successful(self):
+        return self.retcode == ReturnCode.Success
+
+
+class LinearCache:
+    """LinearCache (src/common.jl:281-306).  Assigning `A` marks the cache fresh (refactorize on the
+    next solve_); assigning `b` keeps the device-resident factors / Krylov workspace
+    (src/common.jl:313-360)."""
+
+    def __init__(self, A, b, u, alg, Pl, Pr, abstol, reltol, maxiters, verbose, ctx):
+        object.__setattr__(self, "_A", A)
+        object.__setattr__(self, "isfresh", True)
+        self.b = b
+        self.u = u
+        self.alg = alg
+        self.Pl = Pl
+        self.Pr = Pr
+        self.abstol = abstol
+        self.reltol = reltol
+        self.maxiters = maxiters
+        self.verbose = verbose
+        self.ctx = ctx
+        self.cacheval = None
+
+    @property
+    def A(self):
+        return self._A
+
+    @A.setter
+    def A(self, value):
+        object.__setattr__(self, "_A", value)
+        object.__setattr__(self, "isfresh", True)
+
+
+def init(prob, alg, *, alias_A=False, alias_b=False, abstol=None, reltol=None, maxiters=None,
+         Pl=None, Pr=None, verbose=False, device=0):
+    """SciMLBase.init -> __init (src/common.jl:758-931): copies A and b unless aliased (dense and
+    Krylov algorithms default to no aliasing, :525-544; simplegmres.jl:185-186), zero u, default
+    tolerances sqrt(eps) (:411) and maxiters = length(b) (:764)."""
+    A = prob.A if alias_A else (_copy(prob.A) if not isinstance(prob.A, (CSRMatrix, BlockDiagonal)) else prob.A)
+    b = prob.b if alias_b else _copy(prob.b)
+    if isinstance(alg, MI355XSimpleGMRES) and len(b.shape) == 2:
+        # src/common.jl:614-625: Krylov methods take a vector right-hand side only
+        raise ValueError("SimpleGMRES does not support a matrix right-hand side b")
+    u = _empty_like_rhs(b)
+    if _is_torch(u):
+        u.zero_()
+    else:
+        u[...] = 0.0
+    tol = default_tol()
+    n = b.shape[0]
+    cache = LinearCache(A, b, u, alg, Pl, Pr, tol if abstol is None else abstol,
+                        tol if reltol is None else reltol, n if maxiters is None else maxiters,
+                        verbose, Context.get(device))
+    cache.cacheval = alg.init_cacheval(cache)
+    return cache
+
+
+def solve_(cache):
+    """solve!(cache) (src/common.jl:1012-1017) -> solve!(cache, cache.alg)."""
+    return cache.alg.solve_(cache)
+
+
+def solve(prob, alg, **kwargs):
+    """solve(prob, alg) = solve!(init(prob, alg)) (src/common.jl:988-993)."""
+    return solve_(init(prob, alg, **kwargs))
+
+
+# -------------------------------------------------------------------------------------------------
+# dense LU
+
+class _LUHandle:
+    def __init__(self, ctx, n, dtype):
+        self.ctx = ctx
+        self.n = n
+        self.dtype = dtype
+        h = ctypes.c_void_p()
+        ctx.check(_abi.lib().lsx_lu_create(ctx.handle, n, dtype, ctypes.byref(h)), "lsx_lu_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi.lib().lsx_lu_destroy(h)
+            self.h = None
+
+    def factor(self, A):
+        keep, p = _colmajor(A)
+        return self.ctx.check(_abi.lib().lsx_lu_factor(self.h, p, self.n), "lsx_lu_factor")
+
+    def solve(self, b, u):
+        keep, p = _colmajor(b)
+        nr = _ncols(b)
+        if _is_torch(u) and u.dim() == 2:
+            up, ldu = u.data_ptr(), self.n  # u is a column-major view
+        else:
+            up, ldu = _ptr_out(u), self.n
+        self.ctx.check(_abi.lib().lsx_lu_solve(self.h, nr, p, self.n, up, ldu), "lsx_lu_solve")
+        return u
+
+    def factors(self):
+        F = np.empty((self.n, self.n), order="F")
+        ipiv = np.empty(self.n, dtype=np.int64)
+        self.ctx.check(_abi.lib().lsx_lu_get_factors(self.h, F.ctypes.data, self.n, ipiv.ctypes.data),
+                       "lsx_lu_get_factors")
+        return F, ipiv
+
+    def refine(self, b, u, tol, max_iters, mode):
+        keep, p = _colmajor(b)
+        nr = _ncols(b)
+        st = _abi.RefineStats()
+        up = u.data_ptr() if _is_torch(u) else _ptr_out(u)
+        self.ctx.check(_abi.lib().lsx_lu_refine(self.h, nr, p, self.n, up, self.n, tol, max_iters, mode,
+                                                ctypes.byref(st)), "lsx_lu_refine")
+        return st
+
+
+class _DenseAlg:
+    needs_concrete_A = True
+    needs_square_A = True
+    dtype = LSX_F64
+
+    def init_cacheval(self, cache):
+        # src/factorization.jl:840-849: a placeholder; the factorization happens in solve!
+        return None
+
+    def _fresh_handle(self, cache, n):
+        cv = cache.cacheval
+        if cv is None or cv.n != n or cv.dtype != self.dtype:
+            cv = _LUHandle(cache.ctx, n, self.dtype)
+            cache.cacheval = cv
+        return cv
+
+
+class MI355XLUFactorization(_DenseAlg):
+    """Drop-in for LUFactorization (src/factorization.jl:306-314): FP64 partial-pivot getrf on
+    gfx950, factors and pivots resident on the device until `cache.A` changes."""
+
+    def __init__(self, pivot="RowMaximum", residualsafety=False):
+        if pivot != "RowMaximum":
+            raise ValueError("only RowMaximum pivoting is implemented")
+        self.pivot = pivot
+        self.residualsafety = residualsafety
+
+    def solve_(self, cache):
+        A = cache.A
+        n = A.shape[0]
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("MI355XLUFactorization needs a square A")
+        if cache.isfresh:
+            h = self._fresh_handle(cache, n)
+            info = h.factor(A)
+            if info > 0:
+                # src/factorization.jl:716-722: Failure, cache stays fresh
+                return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
+            object.__setattr__(cache, "isfresh", False)
+        cache.cacheval.solve(cache.b, cache.u)
+        if self.residualsafety:
+            # src/factorization.jl:127-156
+            r = _np(cache.A) @ _np(cache.u) - _np(cache.b)
+            if np.linalg.norm(r) > cache.abstol + cache.reltol * np.linalg.norm(_np(cache.b)):
+                return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
+        return LinearSolution(cache.u, None, ReturnCode.Success, 0, cache)
+
+
+class MI355XMixedLUFactorization(_DenseAlg):
+    """Mixed-precision LU: factor in FP32 or BF16 (FP32 accumulate) on the GPU, recover FP64
+    accuracy by iterative refinement or GMRES-IR with FP64 residuals.  API shape of
+    OpenBLAS32MixedLUFactorization (src/openblas.jl:461-543), which stops at FP32 accuracy."""
+
+    def __init__(self, precision="fp32", refinement="auto", tol=1e-13, max_iters=50):
+        if precision not in ("fp32", "bf16"):
+            raise ValueError("precision must be 'fp32' or 'bf16'")
+        self.precision = precision
+        self.dtype = LSX_F32 if precision == "fp32" else LSX_BF16
+        self.mode = {"ir": 0, "gmres-ir": 1, "auto": 2}[refinement]
+        self.tol = tol
+        self.max_iters = max_iters
+        self.last_stats = None
+
+    def solve_(self, cache):
+        A = cache.A
+        n = A.shape[0]
+        if cache.isfresh:
+            h = self._fresh_handle(cache, n)
+            info = h.factor(A)
+            if info > 0:
+                return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
+            object.__setattr__(cache, "isfresh", False)
+        st = cache.cacheval.refine(cache.b, cache.u, self.tol, self.max_iters, self.mode)
+        self.last_stats = st
+        rc = ReturnCode.Success if st.converged else ReturnCode.MaxIters
+        return LinearSolution(cache.u, st.backward_error, rc, st.iterations, cache,
+                              stats=dict(mode_used=st.mode_used, iterations=st.iterations, outer=st.outer,
+                                         backward_error=st.backward_error))
+
+
+def _np(x):
+    if _is_torch(x):
+        return x.detach().cpu().numpy()
+    return np.asarray(x)
+
+
+# -------------------------------------------------------------------------------------------------
+# batched (block-diagonal) LU
+
+class BlockDiagonal:
+    """Square blocks on the diagonal (the BlockDiagonals.jl matrix type the reference's blockwise
+    factorization takes)."""
+
+    def __init__(self, blocks):
+        self.blocks = [np.asarray(B, dtype=np.float64) for B in blocks]
+        for B in self.blocks:
+            if B.ndim != 2 or B.shape[0] != B.shape[1]:
+                raise ValueError("blocks must be square")
+        self.sizes = np.array([B.shape[0] for B in self.blocks], dtype=np.int64)
+        n = int(self.sizes.sum())
+        self.shape = (n, n)
+
+    @classmethod
+    def from_packed(cls, sizes, packed):
+        obj = cls.__new__(cls)
+        obj.sizes = np.asarray(sizes, dtype=np.int64)
+        obj._packed = np.ascontiguousarray(packed, dtype=np.float64)
+        obj.blocks = None
+        n = int(obj.sizes.sum())
+        obj.shape = (n, n)
+        return obj
+
+    def packed(self):
+        """Blocks column-major, back to back (the lsx_batch layout)."""
+        if getattr(self, "_packed", None) is None:
+            self._packed = np.concatenate([np.asfortranarray(B).ravel(order="F") for B in self.blocks])
+        return self._packed
+
+    def to_dense(self):
+        n = self.shape[0]
+        D = np.zeros((n, n))
+        off = 0
+        P = self.packed()
+        po = 0
+        for s in self.sizes:
+            s = int(s)
+            D[off:off + s, off:off + s] = P[po:po + s * s].reshape((s, s), order="F")
+            off += s
+            po += s * s
+        return D
+
+
+class _BatchHandle:
+    def __init__(self, ctx, sizes, dtype):
+        self.ctx = ctx
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+        h = ctypes.c_void_p()
+        ctx.check(_abi.lib().lsx_batch_create(ctx.handle, len(self.sizes), self.sizes.ctypes.data, dtype,
+                                              ctypes.byref(h)), "lsx_batch_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi.lib().lsx_batch_destroy(h)
+            self.h = None
+
+    def factor_solve(self, Apacked, b, x, want_pivots=False):
+        info = np.zeros(len(self.sizes), dtype=np.int64)
+        ipiv = np.zeros(int(self.sizes.sum()), dtype=np.int64) if want_pivots else None
+        ka, pa = _colmajor(Apacked)
+        kb, pb = _colmajor(b)
+        nfail = self.ctx.check(_abi.lib().lsx_batch_factor_solve(
+            self.h, pa, pb, _ptr_out(x), info.ctypes.data, ipiv.ctypes.data if want_pivots else None),
+            "lsx_batch_factor_solve")
+        return nfail, info, ipiv
+
+
+class MI355XBatchedLUFactorization(_DenseAlg):
+    """Drop-in for the blockwise LU of BlockDiagonalFactorization
+    (ext/LinearSolveBlockDiagonalsExt.jl:48-185): every block factored and solved in one
+    persistent launch.  success = all blocks nonsingular (:101-114)."""
+
+    def __init__(self, precision="fp64"):
+        self.dtype = LSX_F64 if precision == "fp64" else LSX_F32
+        self.last_info = None
+        self.last_ipiv = None
+
+    def solve_(self, cache):
+        A = cache.A
+        if not isinstance(A, BlockDiagonal):
+            raise ValueError("MI355XBatchedLUFactorization takes a BlockDiagonal A")
+        if cache.isfresh or cache.cacheval is None:
+            cache.cacheval = _BatchHandle(cache.ctx, A.sizes, self.dtype)
+            object.__setattr__(cache, "isfresh", False)
+        nfail, info, ipiv = cache.cacheval.factor_solve(A.packed(), cache.b, cache.u, want_pivots=True)
+        self.last_info, self.last_ipiv = info, ipiv
+        rc = ReturnCode.Success if nfail == 0 else ReturnCode.Failure
+        return LinearSolution(cache.u, None, rc, 0, cache)
+
+
+# -------------------------------------------------------------------------------------------------
+# SimpleGMRES
+
+class CSRMatrix:
+    """Sparse matrix in CSR form (int32 indices, FP64 values)."""
+
+    def __init__(self, rowptr, colind, vals, shape):
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.int32)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.shape = tuple(shape)
+
+    @classmethod
+    def from_scipy(cls, S):
+        S = S.tocsr()
+        S.sort_indices()
+        return cls(S.indptr, S.indices, S.data, S.shape)
+
+    @property
+    def nnz(self):
+        return int(self.vals.shape[0])
+
+    def diagonal(self):
+        d = np.zeros(self.shape[0])
+        for i in range(self.shape[0]):
+            lo, hi = self.rowptr[i], self.rowptr[i + 1]
+            m = self.colind[lo:hi] == i
+            if m.any():
+                d[i] = self.vals[lo:hi][m][0]
+        return d
+
+
+class _CSRHandle:
+    def __init__(self, ctx, A):
+        self.ctx = ctx
+        self.n = A.shape[0]
+        h = ctypes.c_void_p()
+        ctx.check(_abi.lib().lsx_csr_create(ctx.handle, A.shape[0], A.nnz, A.rowptr.ctypes.data,
+                                            A.colind.ctypes.data, A.vals.ctypes.data, ctypes.byref(h)),
+                  "lsx_csr_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _abi._lib is not None:
+            _abi.lib().lsx_csr_destroy(h)
+            self.h = None
+
+    def spmv(self, x):
+        kx, px = _colmajor(x)
+        y = _empty_like_rhs(x)
+        self.ctx.check(_abi.lib().lsx_csr_spmv(self.h, px, _ptr_out(y)), "lsx_csr_spmv")
+        return y
+
+
+def _diag_of(P, n):
+    """Diagonal preconditioner as a vector (Pl = Diagonal(d)); None for the identity."""
+    if P is None:
+        return None
+    if isinstance(P, (int, float)) and P == 1:
+        return None
+    d = np.ascontiguousarray(P, dtype=np.float64).ravel()
+    if d.shape[0] != n:
+        raise ValueError("diagonal preconditioner length mismatch")
+    return d
+
+
+class MI355XSimpleGMRES:
+    """Drop-in for SimpleGMRES (src/simplegmres.jl:43-66) on a CSR matrix: restarted GMRES with
+    MGS Arnoldi and symmetric Givens rotations; Krylov workspace resident on the device."""
+
+    needs_concrete_A = True
+    needs_square_A = True
+
+    def __init__(self, restart=True, memory=20, blocksize=0, warm_start=False):
+        if blocksize != 0:
+            raise ValueError("the uniform block-diagonal layout is not implemented")
+        if warm_start:
+            raise ValueError("warm_start is not supported (it does not seed from u0 in the reference)")
+        self.restart = restart
+        self.memory = memory
+
+    def init_cacheval(self, cache):
+        return None
+
+    def solve_(self, cache):
+        A = cache.A
+        if not isinstance(A, CSRMatrix):
+            A = CSRMatrix.from_scipy(A)
+            object.__setattr__(cache, "_A", A)
+        n = A.shape[0]
+        if cache.isfresh or cache.cacheval is None:
+            cache.cacheval = _CSRHandle(cache.ctx, A)
+            object.__setattr__(cache, "isfresh", False)
+        pl = _diag_of(cache.Pl, n)
+        pr = _diag_of(cache.Pr, n)
+        kb, pb = _colmajor(cache.b)
+        st = _abi.GmresStats()
+        cache.ctx.check(_abi.lib().lsx_gmres_solve(
+            cache.cacheval.h, pb, _ptr_out(cache.u), self.memory, int(bool(self.restart)), cache.abstol,
+            cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
+            pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve")
+        return LinearSolution(cache.u, st.resid, ReturnCode(st.retcode), int(st.iters), cache)
+
+
+# -------------------------------------------------------------------------------------------------
+# serving
+
+class Server:
+    """Continuous-batching request loop (lsx_serve_*)."""
+
+    def __init__(self, ctx=None, hbm_budget=64 << 30, nstreams=4):
+        self.ctx = ctx or Context.get(0)
+        h = ctypes.c_void_p()
+        self.ctx.check(_abi.lib().lsx_serve_create(self.ctx.handle, hbm_budget, nstreams, ctypes.byref(h)),
+                       "lsx_serve_create")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            _abi.lib().lsx_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_dense(self, A, b):
+        Af = np.asfortranarray(A, dtype=np.float64)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(bb)
+        req = _abi.Request()
+        req.kind = 0
+        req.dtype = LSX_F64
+        req.n = Af.shape[0]
+        req.A = Af.ctypes.data
+        req.b = bb.ctypes.data
+        t = ctypes.c_uint64()
+        self.ctx.check(_abi.lib().lsx_serve_submit(self.h, ctypes.byref(req), x.ctypes.data, ctypes.byref(t)),
+                       "lsx_serve_submit")
+        self._keep[t.value] = (Af, bb, x)
+        return t.value
+
+    def submit_gmres(self, A, b, pl=None, memory=30, restart=True, abstol=None, reltol=1e-8, maxiters=None):
+        A = A if isinstance(A, CSRMatrix) else CSRMatrix.from_scipy(A)
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(bb)
+        pld = None if pl is None else np.ascontiguousarray(pl, dtype=np.float64)
+        req = _abi.Request()
+        req.kind = 1
+        req.dtype = LSX_F64
+        req.n = A.shape[0]
+        req.b = bb.ctypes.data
+        req.nnz = A.nnz
+        req.rowptr = A.rowptr.ctypes.data
+        req.colind = A.colind.ctypes.data
+        req.vals = A.vals.ctypes.data
+        req.pl_diag = pld.ctypes.data if pld is not None else None
+        req.memory = memory
+        req.restart = int(bool(restart))
+        req.abstol = default_tol() if abstol is None else abstol
+        req.reltol = reltol
+        req.maxiters = A.shape[0] if maxiters is None else maxiters
+        t = ctypes.c_uint64()
+        self.ctx.check(_abi.lib().lsx_serve_submit(self.h, ctypes.byref(req), x.ctypes.data, ctypes.byref(t)),
+                       "lsx_serve_submit")
+        self._keep[t.value] = (A, bb, x, pld)
+        return t.value
+
+    def poll(self, ticket):
+        r = _abi.Result()
+        rc = _abi.lib().lsx_serve_poll(self.h, ticket, ctypes.byref(r))
+        if rc == _abi.LSX_EAGAIN:
+            return None
+        self.ctx.check(rc, "lsx_serve_poll")
+        keep = self._keep.pop(ticket)
+        x = keep[2]
+        return dict(x=x, status=r.status, retcode=ReturnCode(r.retcode), info=r.info, iters=r.iters,
+                    resid=r.resid, t_submit=r.t_submit, t_start=r.t_start, t_done=r.t_done)
+
+    def drain(self):
+        self.ctx.check(_abi.lib().lsx_serve_drain(self.h), "lsx_serve_drain")
+
+    def clock(self):
+        return _abi.lib().lsx_serve_clock(self.h)

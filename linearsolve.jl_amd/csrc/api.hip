@@ -1,0 +1,213 @@
+// api.hip — the C-ABI entry points of liblsx.so (declared in include/lsx.h).
+#include <algorithm>
+#include <new>
+
+#include "lu.hpp"
+
+namespace lsx {
+int fail(lsx_ctx* ctx, int code, const char* fmt, ...) {
+    if (!ctx) return code;
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->err = buf;
+    return code;
+}
+
+int lu64_alloc(lsx_lu* lu) {
+    lsx_ctx* ctx = lu->ctx;
+    const int64_t n = lu->n;
+    lu->ld = pad_ld(n);
+    const int nblk = (int)cdiv(n, TRB);
+    LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream, hipStreamNonBlocking));
+    if (hipMalloc(&lu->F, sizeof(double) * lu->ld * n) != hipSuccess)
+        return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate %lld x %lld FP64 factors", (long long)n,
+                    (long long)n);
+    LSX_HIP(ctx, hipMalloc(&lu->ipiv, sizeof(int64_t) * n));
+    LSX_HIP(ctx, hipMalloc(&lu->perm, sizeof(int) * n));
+    LSX_HIP(ctx, hipMalloc(&lu->dinfo, sizeof(int64_t) * 2));
+    LSX_HIP(ctx, hipHostMalloc(&lu->hinfo, sizeof(int64_t) * 2, hipHostMallocDefault));
+    LSX_HIP(ctx, hipMalloc(&lu->slots, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS));
+    LSX_HIP(ctx, hipMalloc(&lu->plist, sizeof(int64_t) * 8 * NBO));
+    LSX_HIP(ctx, hipMalloc(&lu->pcount, sizeof(int) * 2));
+    LSX_HIP(ctx, hipMalloc(&lu->tflags, sizeof(uint32_t) * 2 * nblk));
+    LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->tflags, 0, sizeof(uint32_t) * 2 * nblk, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo, 0, sizeof(int64_t) * 2, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    return LSX_OK;
+}
+
+void lu64_free(lsx_lu* lu) {
+    if (lu->stream) (void)hipStreamSynchronize(lu->stream);
+    (void)hipFree(lu->F);
+    (void)hipFree(lu->ipiv);
+    (void)hipFree(lu->perm);
+    (void)hipFree(lu->dinfo);
+    if (lu->hinfo) (void)hipHostFree(lu->hinfo);
+    (void)hipFree(lu->slots);
+    (void)hipFree(lu->plist);
+    (void)hipFree(lu->pcount);
+    (void)hipFree(lu->tflags);
+    (void)hipFree(lu->wb);
+    (void)hipFree(lu->wy);
+    if (lu->stream) (void)hipStreamDestroy(lu->stream);
+}
+}  // namespace lsx
+
+using namespace lsx;
+
+extern "C" {
+
+int lsx_version(void) { return 1; }
+
+int lsx_ctx_create(int device, lsx_ctx** out) {
+    if (!out) return LSX_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return LSX_EHIP;
+    if (device < 0 || device >= count) return LSX_EINVAL;
+    lsx_ctx* c = new (std::nothrow) lsx_ctx();
+    if (!c) return LSX_ENOMEM;
+    c->device = device;
+    DeviceGuard g(c);
+    hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        c->wall_khz = khz;
+    *out = c;
+    return LSX_OK;
+}
+
+int lsx_ctx_destroy(lsx_ctx* ctx) {
+    if (!ctx) return LSX_EINVAL;
+    delete ctx;
+    return LSX_OK;
+}
+
+const char* lsx_last_error(const lsx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lsx_malloc(lsx_ctx* ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return LSX_EINVAL;
+    DeviceGuard g(ctx);
+    if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess)
+        return fail(ctx, LSX_ENOMEM, "lsx_malloc: cannot allocate %zu bytes", bytes);
+    return LSX_OK;
+}
+
+int lsx_free(lsx_ctx* ctx, void* dptr) {
+    if (!ctx) return LSX_EINVAL;
+    DeviceGuard g(ctx);
+    LSX_HIP(ctx, hipFree(dptr));
+    return LSX_OK;
+}
+
+int lsx_memcpy(lsx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return LSX_EINVAL;
+    DeviceGuard g(ctx);
+    LSX_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return LSX_OK;
+}
+
+int lsx_synchronize(lsx_ctx* ctx) {
+    if (!ctx) return LSX_EINVAL;
+    DeviceGuard g(ctx);
+    LSX_HIP(ctx, hipDeviceSynchronize());
+    return LSX_OK;
+}
+
+int lsx_lu_create(lsx_ctx* ctx, int64_t n, int dtype, lsx_lu** out) {
+    if (!ctx || !out) return LSX_EINVAL;
+    *out = nullptr;
+    if (n < 1 || n > (int64_t)PMAX * 1024)
+        return fail(ctx, LSX_EINVAL, "lsx_lu_create: n = %lld outside [1, %d]", (long long)n, PMAX * 1024);
+    if (dtype != LSX_F64 && dtype != LSX_F32 && dtype != LSX_BF16)
+        return fail(ctx, LSX_EINVAL, "lsx_lu_create: unknown dtype %d", dtype);
+    DeviceGuard g(ctx);
+    lsx_lu* lu = new (std::nothrow) lsx_lu();
+    if (!lu) return LSX_ENOMEM;
+    lu->ctx = ctx;
+    lu->n = n;
+    lu->dtype = dtype;
+    int rc = lu64_alloc(lu);
+    if (rc == LSX_OK && dtype != LSX_F64) rc = mixed_alloc(lu);
+    if (rc != LSX_OK) {
+        if (dtype != LSX_F64) mixed_free(lu);
+        lu64_free(lu);
+        delete lu;
+        return rc;
+    }
+    *out = lu;
+    return LSX_OK;
+}
+
+int lsx_lu_destroy(lsx_lu* lu) {
+    if (!lu) return LSX_EINVAL;
+    DeviceGuard g(lu->ctx);
+    if (lu->dtype != LSX_F64) mixed_free(lu);
+    lu64_free(lu);
+    delete lu;
+    return LSX_OK;
+}
+
+int lsx_lu_factor(lsx_lu* lu, const double* A, int64_t lda) {
+    if (!lu || !A) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    if (lda < lu->n) return fail(ctx, LSX_EINVAL, "lsx_lu_factor: lda %lld < n %lld", (long long)lda, (long long)lu->n);
+    DeviceGuard g(ctx);
+    lu->factored = false;
+    LSX_HIP(ctx, hipMemcpy2DAsync(lu->F, sizeof(double) * lu->ld, A, sizeof(double) * lda, sizeof(double) * lu->n,
+                                  lu->n, hipMemcpyDefault, lu->stream));
+    int rc = (lu->dtype == LSX_F64) ? lu64_getrf(lu) : mixed_getrf(lu);
+    if (rc < 0) return rc;
+    lu->info = rc;
+    lu->factored = true;
+    return rc;
+}
+
+int lsx_lu_get_factors(lsx_lu* lu, double* F, int64_t ldf, int64_t* ipiv) {
+    if (!lu) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    if (!lu->factored) return fail(ctx, LSX_ESTATE, "lsx_lu_get_factors: no factorization");
+    DeviceGuard g(ctx);
+    if (F) {
+        if (ldf < lu->n) return fail(ctx, LSX_EINVAL, "lsx_lu_get_factors: ldf < n");
+        if (lu->dtype == LSX_F64)
+            LSX_HIP(ctx, hipMemcpy2DAsync(F, sizeof(double) * ldf, lu->F, sizeof(double) * lu->ld,
+                                          sizeof(double) * lu->n, lu->n, hipMemcpyDefault, lu->stream));
+        else {
+            int rc = mixed_get_factors(lu, F, ldf);
+            if (rc) return rc;
+        }
+    }
+    if (ipiv) LSX_HIP(ctx, hipMemcpyAsync(ipiv, lu->ipiv, sizeof(int64_t) * lu->n, hipMemcpyDefault, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    return LSX_OK;
+}
+
+int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx) {
+    if (!lu || !B || !X || nrhs < 0) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    if (!lu->factored) return fail(ctx, LSX_ESTATE, "lsx_lu_solve: no factorization");
+    if (ldb < lu->n || ldx < lu->n) return fail(ctx, LSX_EINVAL, "lsx_lu_solve: ldb/ldx < n");
+    if (nrhs == 0) return LSX_OK;
+    DeviceGuard g(ctx);
+    int rc = lu_ensure_work(lu, nrhs);
+    if (rc) return rc;
+    const int64_t n = lu->n;
+    LSX_HIP(ctx, hipMemcpy2DAsync(lu->wb, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    rc = (lu->dtype == LSX_F64) ? lu64_getrs_dev(lu, nrhs, lu->wb, n, lu->wb, n) : mixed_getrs_dev(lu, nrhs, lu->wb, n, lu->wb, n);
+    if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpy2DAsync(X, sizeof(double) * ldx, lu->wb, sizeof(double) * n, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t), hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_solve: a block wait exceeded its bound");
+    return LSX_OK;
+}
+
+}  // extern "C"

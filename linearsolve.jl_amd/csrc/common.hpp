@@ -1,0 +1,86 @@
+// common.hpp — shared definitions of the gfx950 dense-LU / SimpleGMRES library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include "lsx.h"
+
+struct lsx_ctx {
+    int device = 0;
+    int num_cus = 0;
+    int64_t wall_khz = 100000;  // device wall clock rate (kHz), for bounded device-side waits
+    std::mutex mu;
+    std::string err;
+};
+
+namespace lsx {
+
+// Records an error message on the context and returns code.
+int fail(lsx_ctx* ctx, int code, const char* fmt, ...);
+
+// Makes the context's device current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(const lsx_ctx* c) {
+        (void)hipGetDevice(&prev);
+        if (c && prev != c->device) (void)hipSetDevice(c->device);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+#define LSX_HIP(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return ::lsx::fail((ctx), LSX_EHIP, "%s failed: %s (%s:%d)", #call,            \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
+
+// Leading dimension of resident matrices: a multiple of 32 elements, nudged off multiples of
+// 512 so that column strides do not all map to the same HBM channel.
+inline int64_t pad_ld(int64_t n) {
+    int64_t ld = rup(n < 1 ? 1 : n, 32);
+    if (ld % 512 == 0) ld += 32;
+    return ld;
+}
+
+// Device-side helpers ---------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ld_relaxed_u64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed_u64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_relaxed_u32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_coherent_f64(const double* p) {
+    return __longlong_as_double(
+        (long long)__hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+// Flag-in-word message (low 32 bits payload, high 32 bits sequence number): a reader that sees
+// the expected sequence number in a 64-bit word also sees that word's payload.
+__device__ __forceinline__ uint64_t ll_pack(uint32_t payload, uint32_t seq) {
+    return ((uint64_t)seq << 32) | payload;
+}
+
+// Stream-ordered device allocation that releases on scope exit unless released.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace lsx

@@ -1,0 +1,60 @@
+// lu.hpp — resident dense-LU handle and the FP64 getrf/getrs driver entry points.
+#pragma once
+#include "common.hpp"
+
+namespace lsx {
+constexpr int PW = 32;        // panel width held in registers by the panel kernel
+constexpr int NBO = 128;      // outer block: K of the trailing MFMA update
+constexpr int PMAX = 64;      // max workgroups cooperating on one panel (1024 rows each)
+constexpr int REC = 4 + 2 * PW;   // words of one exchange record: key lo/hi, row, pad, data
+constexpr int RECS = 72;          // record stride in the exchange buffer (words)
+constexpr int TRB = 64;           // row block of the triangular solves
+constexpr int TRHS = 16;          // right-hand sides per triangular-solve launch
+}  // namespace lsx
+
+struct lsx_lu {
+    lsx_ctx* ctx = nullptr;
+    int64_t n = 0;
+    int dtype = LSX_F64;
+    int64_t ld = 0;
+    hipStream_t stream = nullptr;
+    // FP64 factors (also the FP64 copy of A for the mixed path's residuals)
+    double* F = nullptr;
+    int64_t* ipiv = nullptr;     // device, 1-based
+    int* perm = nullptr;         // device: row i of P*A is row perm[i] of A
+    int64_t* dinfo = nullptr;    // device: [0] info, [1] error flags
+    int64_t* hinfo = nullptr;    // pinned host mirror of dinfo
+    uint64_t* slots = nullptr;   // panel exchange records
+    int64_t* plist = nullptr;    // permutation lists: inner [0, 4*NBO), outer [4*NBO, 8*NBO)
+    int* pcount = nullptr;       // [0] inner T, [1] outer T
+    uint32_t* tflags = nullptr;  // triangular-solve block flags (2 * nblk)
+    uint32_t seq = 1;
+    uint32_t tseq = 1;
+    // solve workspaces
+    double* wb = nullptr;
+    double* wy = nullptr;
+    int64_t wcap = 0;  // columns of wb/wy
+    // mixed precision state
+    float* F32 = nullptr;         // low-precision working factors (FP32 storage)
+    int64_t ld32 = 0;
+    double* rwork = nullptr;      // residual / correction workspace (n x cap)
+    void* aux = nullptr;          // mixed-path extra state
+    bool factored = false;
+    int64_t info = 0;
+};
+
+namespace lsx {
+int lu64_alloc(lsx_lu* lu);
+void lu64_free(lsx_lu* lu);
+// Factors lu->F in place (already holding A).  Returns info >= 0 or an lsx_status < 0.
+int lu64_getrf(lsx_lu* lu);
+// X = A \ B with device buffers (B n x nrhs, ldb; X n x nrhs, ldx; may alias).
+int lu64_getrs_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx);
+int lu_ensure_work(lsx_lu* lu, int64_t nrhs);
+// Mixed-precision (FP32 / BF16) factorization state, lu_mixed.hip.
+int mixed_alloc(lsx_lu* lu);
+void mixed_free(lsx_lu* lu);
+int mixed_getrf(lsx_lu* lu);
+int mixed_get_factors(lsx_lu* lu, double* F, int64_t ldf);
+int mixed_getrs_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx);
+}  // namespace lsx
