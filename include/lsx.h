@@ -88,6 +88,21 @@ int lsx_lu_get_factors(lsx_lu* lu, double* F, int64_t ldf, int64_t* ipiv);
  * (ccall path src/openblas.jl:267-275). */
 int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
 
+/* Per-kernel-class device timing of a handle (HIP events around each launch while enabled).
+ * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update, 3 permutation lists,
+ * 4 triangular solves, 5 refinement residual/update, 6 low-precision conversion, 7 other.
+ * flops/bytes are the algorithmic work of the launches timed. */
+typedef struct {
+    int64_t launches[8];
+    double ms[8];
+    double flops[8];
+    double bytes[8];
+} lsx_profile;
+
+/* enable = 1 starts timing (and clears the counters), 0 stops; out (nullable) receives the
+ * counters accumulated so far. */
+int lsx_lu_profile(lsx_lu* lu, int32_t enable, lsx_profile* out);
+
 /* Statistics of the last lsx_lu_refine call. */
 typedef struct {
     int32_t mode_used;      /* 0 = iterative refinement, 1 = GMRES-IR */

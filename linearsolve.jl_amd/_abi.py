@@ -35,6 +35,14 @@ class RefineStats(ctypes.Structure):
                 ("converged", ctypes.c_int32), ("backward_error", ctypes.c_double)]
 
 
+class Profile(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int64 * 8), ("ms", ctypes.c_double * 8), ("flops", ctypes.c_double * 8),
+                ("bytes", ctypes.c_double * 8)]
+
+
+PROFILE_CLASSES = ["panel", "laswp_trsm", "gemm", "perm", "trsv", "refine", "convert", "other"]
+
+
 class GmresStats(ctypes.Structure):
     _fields_ = [("retcode", ctypes.c_int32), ("pad", ctypes.c_int32), ("iters", ctypes.c_int64),
                 ("resid", ctypes.c_double)]
@@ -73,6 +81,7 @@ SIGNATURES = [
     ("lsx_lu_factor", ctypes.c_int, [_P, _P, _I64]),
     ("lsx_lu_get_factors", ctypes.c_int, [_P, _P, _I64, _P]),
     ("lsx_lu_solve", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64]),
+    ("lsx_lu_profile", ctypes.c_int, [_P, _I32, ctypes.POINTER(Profile)]),
     ("lsx_lu_refine", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _I32, _I32, ctypes.POINTER(RefineStats)]),
     ("lsx_batch_create", ctypes.c_int, [_P, _I64, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     ("lsx_batch_destroy", ctypes.c_int, [_P]),

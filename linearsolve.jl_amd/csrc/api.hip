@@ -21,7 +21,7 @@ int lu64_alloc(lsx_lu* lu) {
     lsx_ctx* ctx = lu->ctx;
     const int64_t n = lu->n;
     lu->ld = pad_ld(n);
-    const int nblk = (int)cdiv(n, TRB);
+    const int nblk = (int)cdiv(n, DB);
     LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream, hipStreamNonBlocking));
     if (hipMalloc(&lu->F, sizeof(double) * lu->ld * n) != hipSuccess)
         return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate %lld x %lld FP64 factors", (long long)n,
@@ -34,6 +34,8 @@ int lu64_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&lu->plist, sizeof(int64_t) * 8 * NBO));
     LSX_HIP(ctx, hipMalloc(&lu->pcount, sizeof(int) * 2));
     LSX_HIP(ctx, hipMalloc(&lu->tflags, sizeof(uint32_t) * 2 * nblk));
+    LSX_HIP(ctx, hipMalloc(&lu->DL, sizeof(double) * DB * DB * nblk));
+    LSX_HIP(ctx, hipMalloc(&lu->DU, sizeof(double) * DB * DB * nblk));
     LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, lu->stream));
     LSX_HIP(ctx, hipMemsetAsync(lu->tflags, 0, sizeof(uint32_t) * 2 * nblk, lu->stream));
     LSX_HIP(ctx, hipMemsetAsync(lu->dinfo, 0, sizeof(int64_t) * 2, lu->stream));
@@ -52,6 +54,8 @@ void lu64_free(lsx_lu* lu) {
     (void)hipFree(lu->plist);
     (void)hipFree(lu->pcount);
     (void)hipFree(lu->tflags);
+    (void)hipFree(lu->DL);
+    (void)hipFree(lu->DU);
     (void)hipFree(lu->wb);
     (void)hipFree(lu->wy);
     if (lu->stream) (void)hipStreamDestroy(lu->stream);
@@ -74,7 +78,7 @@ int lsx_ctx_create(int device, lsx_ctx** out) {
     if (!c) return LSX_ENOMEM;
     c->device = device;
     DeviceGuard g(c);
-    hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    (void)hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->wall_khz = khz;
@@ -122,8 +126,8 @@ int lsx_synchronize(lsx_ctx* ctx) {
 int lsx_lu_create(lsx_ctx* ctx, int64_t n, int dtype, lsx_lu** out) {
     if (!ctx || !out) return LSX_EINVAL;
     *out = nullptr;
-    if (n < 1 || n > (int64_t)PMAX * 1024)
-        return fail(ctx, LSX_EINVAL, "lsx_lu_create: n = %lld outside [1, %d]", (long long)n, PMAX * 1024);
+    if (n < 1 || n > (int64_t)PMAX * 512)
+        return fail(ctx, LSX_EINVAL, "lsx_lu_create: n = %lld outside [1, %d]", (long long)n, PMAX * 512);
     if (dtype != LSX_F64 && dtype != LSX_F32 && dtype != LSX_BF16)
         return fail(ctx, LSX_EINVAL, "lsx_lu_create: unknown dtype %d", dtype);
     DeviceGuard g(ctx);
@@ -206,7 +210,19 @@ int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double*
                                   hipMemcpyDefault, lu->stream));
     LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t), hipMemcpyDeviceToHost, lu->stream));
     LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    lu->prof.collect();
     if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_solve: a block wait exceeded its bound");
+    return LSX_OK;
+}
+
+int lsx_lu_profile(lsx_lu* lu, int32_t enable, lsx_profile* out) {
+    if (!lu) return LSX_EINVAL;
+    DeviceGuard g(lu->ctx);
+    LSX_HIP(lu->ctx, hipStreamSynchronize(lu->stream));
+    lu->prof.collect();
+    if (out) *out = lu->prof.acc;
+    if (enable && !lu->prof.on) lu->prof.reset();
+    lu->prof.on = enable != 0;
     return LSX_OK;
 }
 

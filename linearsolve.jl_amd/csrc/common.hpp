@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "lsx.h"
 
@@ -76,11 +77,62 @@ __device__ __forceinline__ uint64_t ll_pack(uint32_t payload, uint32_t seq) {
     return ((uint64_t)seq << 32) | payload;
 }
 
-// Stream-ordered device allocation that releases on scope exit unless released.
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
+// Per-kernel-class device timing: HIP events around launches while enabled, read back after
+// the owning stream has been synchronized.
+enum { PK_PANEL = 0, PK_LASWP = 1, PK_GEMM = 2, PK_PERM = 3, PK_TRSV = 4, PK_REFINE = 5, PK_CONVERT = 6,
+       PK_OTHER = 7 };
+struct Profiler {
+    bool on = false;
+    lsx_profile acc{};
+    struct Rec {
+        hipEvent_t s, e;
+        int cls;
+        double flops, bytes;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    int begin(hipStream_t st, int cls, double flops, double bytes) {
+        if (!on) return -1;
+        Rec r{get(), get(), cls, flops, bytes};
+        (void)hipEventRecord(r.s, st);
+        pending.push_back(r);
+        return (int)pending.size() - 1;
+    }
+    void end(hipStream_t st, int idx) {
+        if (idx >= 0) (void)hipEventRecord(pending[idx].e, st);
+    }
+    void collect() {
+        for (auto& r : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.s, r.e) == hipSuccess) {
+                acc.launches[r.cls] += 1;
+                acc.ms[r.cls] += ms;
+                acc.flops[r.cls] += r.flops;
+                acc.bytes[r.cls] += r.bytes;
+            }
+            pool.push_back(r.s);
+            pool.push_back(r.e);
+        }
+        pending.clear();
+    }
+    void reset() { acc = lsx_profile{}; }
+    ~Profiler() {
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.s);
+            (void)hipEventDestroy(r.e);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
 };
 
 }  // namespace lsx

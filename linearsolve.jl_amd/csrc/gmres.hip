@@ -1,0 +1,590 @@
+// gmres.hip — SimpleGMRES (restarted GMRES, modified Gram-Schmidt Arnoldi, symmetric Givens)
+// on a CSR matrix, one persistent kernel per solve.
+//
+// Follows src/simplegmres.jl (scalar layout) step for step:
+//   reinit / _init_cacheval (:110-129, :420-462): x = 0, r0 = Pl \ b, beta = ||r0||,
+//     eps = abstol + reltol * beta;
+//   outer restart loop (:500-601): on passes >= 2, w = b - A x, r0 = Pl \ w;
+//   Arnoldi step (:521-574): q = Pl \ (A (Pr \ v_k)); MGS h_i = <v_i, q>, q -= h_i v_i (i = 1..k);
+//     Hbis = ||q||; previous Givens applied to the new column, new Givens (_sym_givens
+//     :140-162); zeta update; stopping tests rNorm <= eps, rNorm + 1 <= 1, Hbis <= eps^(3/4);
+//     v_{k+1} = q / Hbis;
+//   back-substitution R y = z with zero for |R_ii| <= btol (:576-585), x_r += sum y_i v_i,
+//     x_r = Pr \ x_r, x += x_r (:587-595); retcodes (:603-606).
+//
+// Layout: workgroup w of P owns rows [w R, (w+1) R), R = 1024 * RPT; thread t holds rows
+// w R + t + i 1024 (i < RPT) of q and of the current basis vector in registers.  Every
+// workgroup keeps its own copy of the Hessenberg/Givens scalars (R, c, s, z) in LDS and
+// computes them identically from all-reduced values, so all workgroups take the same branches.
+// All-reduces combine per-workgroup partial sums in a fixed order (deterministic, the same
+// result in every workgroup) through flag-in-word messages; basis vectors written by one
+// workgroup and read by the SpMV of another are ordered by a counter barrier with agent-scope
+// release/acquire fences.
+#include <algorithm>
+#include <cfloat>
+#include <new>
+
+#include "gmres.hpp"
+
+namespace lsx {
+
+constexpr int GBS = 512;            // threads per workgroup
+constexpr int GNW = GBS / 64;
+
+struct GmresArgs {
+    int64_t N;
+    const int32_t* rowptr;
+    const int32_t* col;
+    const double* val;
+    const double* b;
+    double* x;
+    const double* pl;   // nullable: Jacobi / diagonal left preconditioner
+    const double* pr;   // nullable: diagonal right preconditioner
+    double* V;          // (cap + 1) x N basis
+    double* xr;         // N: correction of the current pass (restart)
+    int memory;
+    int restart;
+    int cap;            // size of the LDS bookkeeping (memory, or maxiters when not restarted)
+    int64_t maxiters;
+    double abstol, reltol;
+    int P;
+    uint64_t* red;      // LL all-reduce slots: 2 parities x P x 2 words
+    uint32_t* bar;      // barrier counter
+    GmresOut* out;
+    int64_t timeout;
+};
+
+struct GShared {
+    double wsum[2][GNW];
+    uint32_t rbuf[2][2 * 256];  // 2 words per workgroup, up to 256 workgroups
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <int RPT>
+struct GmresCtx {
+    const GmresArgs& a;
+    GShared& sh;
+    int tid, lane, wave, w;
+    int64_t row0;     // first row of this workgroup
+    uint32_t nred = 0, nbar = 0;
+
+    __device__ GmresCtx(const GmresArgs& a_, GShared& sh_) : a(a_), sh(sh_) {
+        tid = threadIdx.x;
+        lane = tid & 63;
+        wave = tid >> 6;
+        w = blockIdx.x;
+        row0 = (int64_t)w * GBS * RPT;
+    }
+    __device__ int64_t row(int i) const { return row0 + tid + (int64_t)i * GBS; }
+
+    // Sum of v over all threads of all workgroups, identical in every thread.
+    __device__ double allreduce(double v) {
+        const int par = (int)(nred & 1);
+        ++nred;
+        v = wave_sum(v);
+        if (lane == 0) sh.wsum[par][wave] = v;
+        __syncthreads();
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < GNW; ++i) s += sh.wsum[par][i];
+        if (a.P == 1) return s;
+        const uint32_t seq = nred;
+        uint64_t* slot = a.red + (size_t)par * a.P * 2;
+        if (tid == 0) {
+            st_relaxed_u64(slot + 2 * w, ll_pack((uint32_t)__double2loint(s), seq));
+            st_relaxed_u64(slot + 2 * w + 1, ll_pack((uint32_t)__double2hiint(s), seq));
+        }
+        if (tid < 2 * a.P) {
+            uint64_t x = ld_relaxed_u64(slot + tid);
+            if ((uint32_t)(x >> 32) != seq) {
+                const int64_t t0 = wall_clock64();
+                while (true) {
+                    x = ld_relaxed_u64(slot + tid);
+                    if ((uint32_t)(x >> 32) == seq) break;
+                    if (wall_clock64() - t0 > a.timeout) {
+                        atomicOr((unsigned long long*)&a.out->err, 1ull);
+                        break;
+                    }
+                }
+            }
+            sh.rbuf[par][tid] = (uint32_t)x;
+        }
+        __syncthreads();
+        double tot = 0.0;
+        for (int p = 0; p < a.P; ++p)
+            tot += __hiloint2double((int)sh.rbuf[par][2 * p + 1], (int)sh.rbuf[par][2 * p]);
+        return tot;
+    }
+
+    // Orders this workgroup's global writes before every workgroup's later reads.
+    __device__ void barrier() {
+        if (a.P == 1) {
+            __syncthreads();
+            return;
+        }
+        ++nbar;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            __hip_atomic_fetch_add(a.bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t target = nbar * (uint32_t)a.P;
+            if (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                const int64_t t0 = wall_clock64();
+                while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    if (wall_clock64() - t0 > a.timeout) {
+                        atomicOr((unsigned long long*)&a.out->err, 2ull);
+                        break;
+                    }
+                }
+            }
+            __atomic_thread_fence(__ATOMIC_ACQUIRE);
+        }
+        __syncthreads();
+    }
+
+    // y_r = (A (v ./ pr))_r, terms summed in ascending column order without fusion (the order
+    // in which SparseArrays' CSC mul! accumulates a row).
+    __device__ double spmv_row(int64_t r, const double* v) const {
+        const int32_t k0 = a.rowptr[r], k1 = a.rowptr[r + 1];
+        double s = 0.0;
+        for (int32_t k = k0; k < k1; ++k) {
+            const int32_t j = a.col[k];
+            double xv = v[j];
+            if (a.pr) xv = xv / a.pr[j];
+            s = __dadd_rn(s, __dmul_rn(a.val[k], xv));
+        }
+        return s;
+    }
+};
+
+__device__ void sym_givens(double a, double b, double& c, double& s, double& rho) {
+    auto sgn = [](double t) { return t > 0.0 ? 1.0 : (t < 0.0 ? -1.0 : 0.0); };
+    if (b == 0.0) {
+        c = (a == 0.0) ? 1.0 : sgn(a);
+        s = 0.0;
+        rho = fabs(a);
+    } else if (a == 0.0) {
+        c = 0.0;
+        s = sgn(b);
+        rho = fabs(b);
+    } else if (fabs(b) > fabs(a)) {
+        const double t = a / b;
+        s = sgn(b) / sqrt(1.0 + t * t);
+        c = s * t;
+        rho = b / s;
+    } else {
+        const double t = b / a;
+        c = sgn(a) / sqrt(1.0 + t * t);
+        s = c * t;
+        rho = a / c;
+    }
+}
+
+template <int RPT>
+__global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
+    __shared__ GShared sh;
+    extern __shared__ double dyn[];
+    const int cap = a.cap;
+    double* R = dyn;                          // cap (cap + 1) / 2
+    double* cs = R + (cap * (cap + 1)) / 2;   // cap
+    double* sn = cs + cap;                    // cap
+    double* z = sn + cap;                     // cap + 1
+    // q and the current basis slice, one LDS column per thread (each thread touches only its own
+    // slots, so no synchronization is needed for them)
+    double* qs = z + cap + 1;                 // RPT x GBS
+    double* vs = qs + RPT * GBS;              // RPT x GBS
+    __shared__ int s_flags[4];
+    GmresCtx<RPT> g(a, sh);
+    const int64_t N = a.N;
+    const int tid = g.tid;
+    double* x = a.x;
+    double* xr = a.restart ? a.xr : a.x;
+#define q(i) qs[(i) * GBS + tid]
+#define vk(i) vs[(i) * GBS + tid]
+
+    // x = 0; r0 = Pl \ b
+    double part = 0.0;
+#pragma unroll 2
+    for (int i = 0; i < RPT; ++i) {
+        const int64_t r = g.row(i);
+        q(i) = 0.0;
+        if (r < N) {
+            x[r] = 0.0;
+            q(i) = a.pl ? a.b[r] / a.pl[r] : a.b[r];
+            part += q(i) * q(i);
+        }
+    }
+    const double beta0 = sqrt(g.allreduce(part));
+    const double eps_ = a.abstol + a.reltol * beta0;
+    double rNorm = beta0;
+    int64_t iter = 0;
+    int status = LSX_RET_DEFAULT;
+    if (beta0 == 0.0) {
+        status = LSX_RET_SUCCESS;
+    } else {
+        int npass = 0;
+        const double btol = pow(DBL_EPSILON, 0.75);
+        bool breakdown = false, inconsistent = false;
+        bool solved = rNorm <= eps_;
+        int64_t inner_maxiters = a.maxiters;
+        bool tired = iter >= a.maxiters;
+        while (!(solved || tired || breakdown)) {
+            int64_t nr = 0;
+            if (a.restart) {
+#pragma unroll 2
+                for (int i = 0; i < RPT; ++i) {
+                    const int64_t r = g.row(i);
+                    if (r < N) xr[r] = 0.0;
+                }
+                if (npass >= 1) {
+                    g.barrier();  // x complete in every workgroup before the SpMV reads it
+#pragma unroll 2
+                    for (int i = 0; i < RPT; ++i) {
+                        const int64_t r = g.row(i);
+                        if (r < N) {
+                            double Ax = 0.0;
+                            const int32_t k0 = a.rowptr[r], k1 = a.rowptr[r + 1];
+                            for (int32_t k = k0; k < k1; ++k) Ax = __dadd_rn(Ax, __dmul_rn(a.val[k], x[a.col[k]]));
+                            const double wv = a.b[r] - Ax;
+                            q(i) = a.pl ? wv / a.pl[r] : wv;
+                        }
+                    }
+                }
+            }
+            // zeta_1 = ||r0||, v_1 = r0 / ||r0||
+            part = 0.0;
+#pragma unroll 2
+            for (int i = 0; i < RPT; ++i) part += (g.row(i) < N) ? q(i) * q(i) : 0.0;
+            const double b0 = sqrt(g.allreduce(part));
+            if (tid == 0) z[0] = b0;
+#pragma unroll 2
+            for (int i = 0; i < RPT; ++i) {
+                const int64_t r = g.row(i);
+                if (r < N) a.V[r] = q(i) / b0;
+            }
+            g.barrier();
+            ++npass;
+            int k = 0;
+            bool inner_tired = false;
+            while (!(solved || inner_tired || breakdown)) {
+                ++k;
+                const double* vcur = a.V + (int64_t)(k - 1) * N;
+                // q = Pl \ (A (Pr \ v_k))
+#pragma unroll 2
+                for (int i = 0; i < RPT; ++i) {
+                    const int64_t r = g.row(i);
+                    q(i) = 0.0;
+                    if (r < N) {
+                        const double wv = g.spmv_row(r, vcur);
+                        q(i) = a.pl ? wv / a.pl[r] : wv;
+                    }
+                }
+                // modified Gram-Schmidt, the axpy of basis vector i fused with the dot of i + 1
+                double h = 0.0;
+                for (int ii = 1; ii <= k + 1; ++ii) {
+                    part = 0.0;
+                    const double* vi = a.V + (int64_t)(ii - 1) * N;
+#pragma unroll 2
+                    for (int i = 0; i < RPT; ++i) {
+                        const int64_t r = g.row(i);
+                        if (r < N) {
+                            if (ii > 1) q(i) = q(i) - h * vk(i);
+                            if (ii <= k) {
+                                vk(i) = vi[r];
+                                part += vk(i) * q(i);
+                            } else {
+                                part += q(i) * q(i);
+                            }
+                        }
+                    }
+                    const double red = g.allreduce(part);
+                    if (ii <= k) {
+                        h = red;
+                        if (tid == 0) R[nr + ii - 1] = h;
+                    } else {
+                        h = sqrt(red);  // Hbis
+                    }
+                }
+                const double Hbis = h;
+                if (tid == 0) {
+                    for (int i = 1; i <= k - 1; ++i) {
+                        const double Rtmp = cs[i - 1] * R[nr + i - 1] + sn[i - 1] * R[nr + i];
+                        R[nr + i] = sn[i - 1] * R[nr + i - 1] - cs[i - 1] * R[nr + i];
+                        R[nr + i - 1] = Rtmp;
+                    }
+                    double c_, s_, rho;
+                    sym_givens(R[nr + k - 1], Hbis, c_, s_, rho);
+                    cs[k - 1] = c_;
+                    sn[k - 1] = s_;
+                    R[nr + k - 1] = rho;
+                    const double zeta = s_ * z[k - 1];
+                    z[k - 1] = c_ * z[k - 1];
+                    z[k] = zeta;  // consumed only if the iteration continues
+                }
+                __syncthreads();
+                rNorm = fabs(z[k]);
+                nr += k;
+                const bool mach = (rNorm + 1.0 <= 1.0);
+                const bool lim = rNorm <= eps_;
+                breakdown = Hbis <= btol;
+                solved = lim || mach;
+                inner_tired = a.restart ? (k >= (int64_t)min((int64_t)a.memory, inner_maxiters))
+                                        : (k >= inner_maxiters);
+                if (!(solved || inner_tired || breakdown)) {
+                    double* vn = a.V + (int64_t)k * N;
+#pragma unroll 2
+                    for (int i = 0; i < RPT; ++i) {
+                        const int64_t r = g.row(i);
+                        if (r < N) vn[r] = q(i) / Hbis;
+                    }
+                    g.barrier();
+                }
+            }
+            // back-substitution R y = z (y overwrites z)
+            if (tid == 0) {
+                int inc = 0;
+                for (int i = k; i >= 1; --i) {
+                    int64_t pos = nr + i - k;
+                    for (int j = k; j >= i + 1; --j) {
+                        z[i - 1] = z[i - 1] - R[pos - 1] * z[j - 1];
+                        pos = pos - j + 1;
+                    }
+                    if (fabs(R[pos - 1]) <= btol) {
+                        z[i - 1] = 0.0;
+                        inc = 1;
+                    } else {
+                        z[i - 1] = z[i - 1] / R[pos - 1];
+                    }
+                }
+                s_flags[0] = inc;
+            }
+            __syncthreads();
+            inconsistent = inconsistent || s_flags[0];
+            // x_r += sum_i y_i v_i; x_r = Pr \ x_r; x += x_r
+#pragma unroll 2
+            for (int i = 0; i < RPT; ++i) {
+                const int64_t r = g.row(i);
+                if (r < N) {
+                    double acc = xr[r];
+                    for (int ii = 0; ii < k; ++ii) acc = acc + z[ii] * a.V[(int64_t)ii * N + r];
+                    if (a.pr) acc = acc / a.pr[r];
+                    if (a.restart) x[r] = x[r] + acc;
+                    else xr[r] = acc;
+                }
+            }
+            inner_maxiters -= k;
+            iter += k;
+            tired = iter >= a.maxiters;
+            __syncthreads();
+        }
+        if (tired) status = LSX_RET_MAXITERS;
+        if (solved) status = LSX_RET_SUCCESS;
+        if (inconsistent) status = LSX_RET_INFEASIBLE;
+    }
+#undef q
+#undef vk
+    if (g.w == 0 && tid == 0) {
+        a.out->retcode = status;
+        a.out->iters = iter;
+        a.out->resid = rNorm;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+__global__ void k_spmv(int64_t N, const int32_t* __restrict rowptr, const int32_t* __restrict col,
+                       const double* __restrict val, const double* __restrict xv, double* __restrict y) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double s = 0.0;
+    for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) s = __dadd_rn(s, __dmul_rn(val[k], xv[col[k]]));
+    y[r] = s;
+}
+
+// Rows per thread for a matrix of N rows: a single workgroup up to 8192 rows, then 16 rows per
+// thread over ceil(N / 8192) workgroups.
+static int pick_rpt(int64_t N, int& P) {
+    if (N <= GBS) { P = 1; return 1; }
+    if (N <= 4 * GBS) { P = 1; return 4; }
+    if (N <= 16 * GBS) { P = 1; return 16; }
+    P = (int)cdiv(N, 16 * GBS);
+    return 16;
+}
+
+int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st) {
+    hipLaunchKernelGGL(k_spmv, dim3((unsigned)cdiv(A->N, 256)), dim3(256), 0, st, A->N, A->rowptr, A->col, A->val,
+                       dx, dy);
+    return hipGetLastError() == hipSuccess ? LSX_OK : LSX_EHIP;
+}
+
+int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx, const double* dpl,
+                 const double* dpr, hipStream_t st) {
+    lsx_ctx* ctx = A->ctx;
+    const int64_t N = A->N;
+    int P = 1;
+    const int rpt = pick_rpt(N, P);
+    if (P > PMAX_G) return fail(ctx, LSX_EINVAL, "gmres: N = %lld exceeds %d rows", (long long)N, PMAX_G * 16 * GBS);
+    const int64_t memory = std::max<int64_t>(1, std::min<int64_t>(p.memory, p.maxiters));
+    const int64_t cap = p.restart ? memory : std::max<int64_t>(memory, p.maxiters);
+    if (cap > GMRES_MAX_CAP)
+        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld exceeds %d (restart = %d)", (long long)cap,
+                    GMRES_MAX_CAP, p.restart);
+    // workspace: basis (cap + 1) x N, correction N
+    const size_t need = sizeof(double) * (size_t)N * (size_t)(cap + 2);
+    if (need > A->ws_bytes) {
+        (void)hipFree(A->ws);
+        A->ws = nullptr;
+        A->ws_bytes = 0;
+        if (hipMalloc(&A->ws, need) != hipSuccess)
+            return fail(ctx, LSX_ENOMEM, "gmres: cannot allocate %zu bytes of Krylov workspace", need);
+        A->ws_bytes = need;
+    }
+    LSX_HIP(ctx, hipMemsetAsync(A->red, 0, sizeof(uint64_t) * 4 * PMAX_G + 256, st));
+    LSX_HIP(ctx, hipMemsetAsync(A->out, 0, sizeof(GmresOut), st));
+    GmresArgs g;
+    g.N = N;
+    g.rowptr = A->rowptr;
+    g.col = A->col;
+    g.val = A->val;
+    g.b = db;
+    g.x = dx;
+    g.pl = dpl;
+    g.pr = dpr;
+    g.V = A->ws;
+    g.xr = A->ws + (size_t)N * (size_t)(cap + 1);
+    g.memory = (int)memory;
+    g.restart = p.restart;
+    g.cap = (int)cap;
+    g.maxiters = p.maxiters;
+    g.abstol = p.abstol;
+    g.reltol = p.reltol;
+    g.P = P;
+    g.red = A->red;
+    g.bar = reinterpret_cast<uint32_t*>(A->red + 4 * PMAX_G);
+    g.out = A->out;
+    g.timeout = ctx->wall_khz * 20000;  // 20 s
+    const size_t dynlds = sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
+    if (dynlds > 150 * 1024)
+        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld too large for N = %lld", (long long)cap, (long long)N);
+    switch (rpt) {
+        case 1: hipLaunchKernelGGL(k_gmres<1>, dim3(P), dim3(GBS), dynlds, st, g); break;
+        case 4: hipLaunchKernelGGL(k_gmres<4>, dim3(P), dim3(GBS), dynlds, st, g); break;
+        default: hipLaunchKernelGGL(k_gmres<16>, dim3(P), dim3(GBS), dynlds, st, g); break;
+    }
+    LSX_HIP(ctx, hipGetLastError());
+    return LSX_OK;
+}
+
+}  // namespace lsx
+
+// ------------------------------------------------------------------------------------------
+// C-ABI
+using namespace lsx;
+
+static void csr_free(lsx_csr* A) {
+    if (A->stream) (void)hipStreamSynchronize(A->stream);
+    (void)hipFree(A->rowptr);
+    (void)hipFree(A->col);
+    (void)hipFree(A->val);
+    (void)hipFree(A->ws);
+    (void)hipFree(A->red);
+    (void)hipFree(A->out);
+    if (A->hout) (void)hipHostFree(A->hout);
+    (void)hipFree(A->db);
+    (void)hipFree(A->dx);
+    (void)hipFree(A->dpl);
+    (void)hipFree(A->dpr);
+    if (A->stream) (void)hipStreamDestroy(A->stream);
+}
+
+extern "C" {
+
+int lsx_csr_create(lsx_ctx* ctx, int64_t N, int64_t nnz, const int32_t* rowptr, const int32_t* colind,
+                   const double* vals, lsx_csr** out) {
+    if (!ctx || !out || !rowptr || (nnz > 0 && (!colind || !vals))) return LSX_EINVAL;
+    *out = nullptr;
+    if (N < 1 || nnz < 0 || nnz > INT32_MAX)
+        return fail(ctx, LSX_EINVAL, "lsx_csr_create: N = %lld, nnz = %lld out of range", (long long)N,
+                    (long long)nnz);
+    DeviceGuard g(ctx);
+    lsx_csr* A = new (std::nothrow) lsx_csr();
+    if (!A) return LSX_ENOMEM;
+    A->ctx = ctx;
+    A->N = N;
+    A->nnz = nnz;
+    auto bad = [&](int rc) {
+        csr_free(A);
+        delete A;
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&A->rowptr, sizeof(int32_t) * (N + 1)) != hipSuccess ||
+        hipMalloc(&A->col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+        hipMalloc(&A->val, sizeof(double) * std::max<int64_t>(nnz, 1)) != hipSuccess ||
+        hipMalloc(&A->red, sizeof(uint64_t) * 4 * PMAX_G + 256) != hipSuccess ||
+        hipMalloc(&A->out, sizeof(GmresOut)) != hipSuccess ||
+        hipHostMalloc(&A->hout, sizeof(GmresOut), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&A->db, sizeof(double) * N) != hipSuccess || hipMalloc(&A->dx, sizeof(double) * N) != hipSuccess ||
+        hipMalloc(&A->dpl, sizeof(double) * N) != hipSuccess || hipMalloc(&A->dpr, sizeof(double) * N) != hipSuccess)
+        return bad(fail(ctx, LSX_ENOMEM, "lsx_csr_create: device allocation failed (N = %lld)", (long long)N));
+    if (hipMemcpyAsync(A->rowptr, rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDefault, A->stream) != hipSuccess ||
+        (nnz > 0 && hipMemcpyAsync(A->col, colind, sizeof(int32_t) * nnz, hipMemcpyDefault, A->stream) != hipSuccess) ||
+        (nnz > 0 && hipMemcpyAsync(A->val, vals, sizeof(double) * nnz, hipMemcpyDefault, A->stream) != hipSuccess) ||
+        hipStreamSynchronize(A->stream) != hipSuccess)
+        return bad(fail(ctx, LSX_EHIP, "lsx_csr_create: upload failed"));
+    *out = A;
+    return LSX_OK;
+}
+
+int lsx_csr_destroy(lsx_csr* A) {
+    if (!A) return LSX_EINVAL;
+    DeviceGuard g(A->ctx);
+    csr_free(A);
+    delete A;
+    return LSX_OK;
+}
+
+int lsx_csr_spmv(lsx_csr* A, const double* x, double* y) {
+    if (!A || !x || !y) return LSX_EINVAL;
+    lsx_ctx* ctx = A->ctx;
+    DeviceGuard g(ctx);
+    LSX_HIP(ctx, hipMemcpyAsync(A->dx, x, sizeof(double) * A->N, hipMemcpyDefault, A->stream));
+    int rc = csr_spmv_dev(A, A->dx, A->db, A->stream);
+    if (rc) return fail(ctx, rc, "lsx_csr_spmv: launch failed");
+    LSX_HIP(ctx, hipMemcpyAsync(y, A->db, sizeof(double) * A->N, hipMemcpyDefault, A->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(A->stream));
+    return LSX_OK;
+}
+
+int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int32_t restart, double abstol,
+                    double reltol, int64_t maxiters, const double* pl_diag, const double* pr_diag,
+                    lsx_gmres_stats* stats) {
+    if (!A || !b || !x) return LSX_EINVAL;
+    lsx_ctx* ctx = A->ctx;
+    if (memory < 1 || maxiters < 0) return fail(ctx, LSX_EINVAL, "lsx_gmres_solve: memory >= 1, maxiters >= 0");
+    DeviceGuard g(ctx);
+    const size_t vb = sizeof(double) * A->N;
+    LSX_HIP(ctx, hipMemcpyAsync(A->db, b, vb, hipMemcpyDefault, A->stream));
+    if (pl_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpl, pl_diag, vb, hipMemcpyDefault, A->stream));
+    if (pr_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpr, pr_diag, vb, hipMemcpyDefault, A->stream));
+    GmresParams p{memory, restart ? 1 : 0, maxiters, abstol, reltol};
+    int rc = gmres_launch(A, p, A->db, A->dx, pl_diag ? A->dpl : nullptr, pr_diag ? A->dpr : nullptr, A->stream);
+    if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpyAsync(x, A->dx, vb, hipMemcpyDefault, A->stream));
+    LSX_HIP(ctx, hipMemcpyAsync(A->hout, A->out, sizeof(GmresOut), hipMemcpyDeviceToHost, A->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(A->stream));
+    if (A->hout->err) return fail(ctx, LSX_ETIMEOUT, "lsx_gmres_solve: a device-side wait exceeded its bound");
+    if (stats) {
+        stats->retcode = A->hout->retcode;
+        stats->pad = 0;
+        stats->iters = A->hout->iters;
+        stats->resid = A->hout->resid;
+    }
+    return LSX_OK;
+}
+
+}  // extern "C"

@@ -5,10 +5,10 @@
 namespace lsx {
 constexpr int PW = 32;        // panel width held in registers by the panel kernel
 constexpr int NBO = 128;      // outer block: K of the trailing MFMA update
-constexpr int PMAX = 64;      // max workgroups cooperating on one panel (1024 rows each)
+constexpr int PMAX = 128;     // max workgroups cooperating on one panel (512 rows each)
 constexpr int REC = 4 + 2 * PW;   // words of one exchange record: key lo/hi, row, pad, data
 constexpr int RECS = 72;          // record stride in the exchange buffer (words)
-constexpr int TRB = 64;           // row block of the triangular solves
+constexpr int DB = 64;            // diagonal block of the explicit inverses and triangular solves
 constexpr int TRHS = 16;          // right-hand sides per triangular-solve launch
 }  // namespace lsx
 
@@ -28,6 +28,8 @@ struct lsx_lu {
     int64_t* plist = nullptr;    // permutation lists: inner [0, 4*NBO), outer [4*NBO, 8*NBO)
     int* pcount = nullptr;       // [0] inner T, [1] outer T
     uint32_t* tflags = nullptr;  // triangular-solve block flags (2 * nblk)
+    double* DL = nullptr;        // inverses of the 64 x 64 diagonal blocks of L (nblk x 64 x 64)
+    double* DU = nullptr;        // ... and of U
     uint32_t seq = 1;
     uint32_t tseq = 1;
     // solve workspaces
@@ -41,6 +43,7 @@ struct lsx_lu {
     void* aux = nullptr;          // mixed-path extra state
     bool factored = false;
     int64_t info = 0;
+    lsx::Profiler prof;
 };
 
 namespace lsx {

@@ -1,12 +1,727 @@
-// lu_mixed.hip — mixed-precision factorization (FP32 / BF16 working precision).
-#include "lu.hpp"
+// lu_mixed.hip — mixed-precision LU: factor in FP32 (FP32 MFMA trailing update) or with BF16
+// trailing updates (BF16 operands with a per-panel power-of-two scale, FP32 accumulation), then
+// recover FP64 accuracy by iterative refinement (IR) or GMRES-IR with FP64 residuals.
+//
+// The API shape follows OpenBLAS32MixedLUFactorization (src/openblas.jl:461-543: cast A to
+// FP32, sgetrf, sgetrs, cast back), which stops at FP32 accuracy; here the low-precision factors
+// precondition an FP64 refinement until the normwise backward error
+//   max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+// reaches the requested tolerance.  FP64 is used only for the residual r = b - A x, the update
+// x += d and (GMRES-IR) the Krylov orthogonalization, as in Carson & Higham (SISC 2018).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lu_driver.hpp"
 
 namespace lsx {
-int mixed_alloc(lsx_lu* lu) { return fail(lu->ctx, LSX_EINVAL, "mixed precision not available yet"); }
-void mixed_free(lsx_lu*) {}
-int mixed_getrf(lsx_lu* lu) { return fail(lu->ctx, LSX_EINVAL, "mixed precision not available yet"); }
-int mixed_get_factors(lsx_lu* lu, double*, int64_t) { return fail(lu->ctx, LSX_EINVAL, "mixed precision not available yet"); }
-int mixed_getrs_dev(lsx_lu* lu, int64_t, const double*, int64_t, double*, int64_t) {
-    return fail(lu->ctx, LSX_EINVAL, "mixed precision not available yet");
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+struct MixedState {
+    float* DL = nullptr;     // FP32 inverses of diagonal blocks
+    float* DU = nullptr;
+    double* anorm = nullptr; // device: ||A||_inf
+    float* scale = nullptr;  // device: per-update power-of-two scale of the U12 operand (BF16 path)
+    double* r = nullptr;     // n x cap residuals / corrections
+    double* xw = nullptr;    // n x cap iterate
+    double* bw = nullptr;    // n x cap right-hand sides
+    double* part = nullptr;  // reduction partials
+    double* hpart = nullptr; // pinned host copy of partials
+    int64_t cap = 0;
+    // GMRES-IR workspace
+    double* V = nullptr;     // n x (m + 1)
+    double* w = nullptr;
+    double* hvec = nullptr;  // device m + 1
+    int gm = 0;
+    double anorm_h = 0.0;
+};
+
+static MixedState* ms(lsx_lu* lu) { return reinterpret_cast<MixedState*>(lu->aux); }
+
+// ---------------------------------------------------------------------------------------- kernels
+__global__ void k_cast_f32(const double* __restrict src, int64_t lds, float* __restrict dst, int64_t ldd, int64_t n) {
+    const int64_t col = blockIdx.y;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[col * ldd + i] = (float)src[col * lds + i];
 }
+
+// Power-of-two scale 2^-e with 2^e >= max |U| over a K x N block (column-major, ld), so the
+// scaled operand has magnitude <= 1 before rounding to BF16.
+__global__ void k_absmax_scale(const float* __restrict U, int64_t ld, int K, int N, float* __restrict out) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int64_t idx = threadIdx.x; idx < (int64_t)K * N; idx += 256) {
+        const int64_t c = idx / K, k = idx - c * K;
+        m = fmaxf(m, fabsf(U[c * ld + k]));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float mx = red[0];
+        int e = 0;
+        if (mx > 0.f && isfinite(mx)) (void)frexpf(mx, &e);
+        *out = ldexpf(1.f, -e);
+    }
+}
+
+constexpr int MBM = 128, MBN = 128;
+
+// C -= A B (mode 0) or C = A B (mode 1, in place on B when M <= 128), FP32 MFMA 16x16x4.
+// Same tiling as the FP64 kernel; lane l / slot q holds C(rb + (l & 15), cb + 4 (l >> 4) + q).
+__global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict Ag, int64_t lda, const float* Bg,
+                                                  int64_t ldb, float* Cg, int64_t ldc, int M, int N, int K,
+                                                  int mode) {
+    constexpr int BK = 16;
+    const int TM = (M + MBM - 1) / MBM, TN = (N + MBN - 1) / MBN, T = TM * TN;
+    const int per = (T + 7) / 8;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= T) return;
+    const int row0 = (tile % TM) * MBM, col0 = (tile / TM) * MBN;
+    __shared__ float As[2][BK][MBM + 4];
+    __shared__ float Bs[2][BK][MBN + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    v4f acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    float ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int idx = it * 256 + tid;
+            const int r = idx & (MBM - 1), k = idx >> 7;
+            ra[it] = (row0 + r < M && k0 + k < K) ? Ag[(int64_t)(k0 + k) * lda + row0 + r] : 0.f;
+            const int kb = idx & (BK - 1), c = idx >> 4;
+            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] : 0.f;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int idx = it * 256 + tid;
+            As[buf][idx >> 7][idx & (MBM - 1)] = ra[it];
+            Bs[buf][idx & (BK - 1)][idx >> 4] = rb[it];
+        }
+    };
+    const int nk = (K + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int s = 0; s < nk; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nk) gload((s + 1) * BK);
+#pragma unroll
+        for (int k4 = 0; k4 < BK / 4; ++k4) {
+            const int kr = k4 * 4 + (lane >> 4);
+            float x[4], y[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = Bs[buf][kr][wn * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) y[j] = As[buf][kr][wm * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i], y[j], acc[i][j], 0, 0, 0);
+        }
+        if (s + 1 < nk) {
+            sstore(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    if (mode == 1) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
+                if (row < M && col < N) {
+                    float* cp = Cg + (int64_t)col * ldc + row;
+                    *cp = (mode == 0) ? (*cp - acc[i][j][q]) : acc[i][j][q];
+                }
+            }
+        }
+}
+
+__device__ __forceinline__ __bf16 to_bf16(float f) {
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    const uint16_t h = (uint16_t)(u >> 16);
+    return __builtin_bit_cast(__bf16, h);
+}
+
+// C -= A B with BF16 operands and FP32 accumulation (v_mfma_f32_16x16x32_bf16).  B (the U12
+// block) is multiplied by the power-of-two *scale before rounding and the product by 1/scale
+// after accumulation (both exact).  A (L21) holds multipliers of magnitude <= 1.
+__global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict Ag, int64_t lda, const float* Bg,
+                                                   int64_t ldb, float* Cg, int64_t ldc, int M, int N, int K,
+                                                   const float* __restrict scale) {
+    constexpr int BK = 32, PAD = 8;
+    const int TM = (M + MBM - 1) / MBM, TN = (N + MBN - 1) / MBN, T = TM * TN;
+    const int per = (T + 7) / 8;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= T) return;
+    const int row0 = (tile % TM) * MBM, col0 = (tile / TM) * MBN;
+    __shared__ __align__(16) __bf16 As[2][MBM][BK + PAD];   // As[row][k]
+    __shared__ __align__(16) __bf16 Bs[2][MBN][BK + PAD];   // Bs[col][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const float s = *scale, inv = 1.f / s;
+    v4f acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    float ra[16], rb[16];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int idx = it * 256 + tid;
+            const int r = idx & (MBM - 1), k = idx >> 7;          // A: rows fastest (coalesced)
+            ra[it] = (row0 + r < M && k0 + k < K) ? Ag[(int64_t)(k0 + k) * lda + row0 + r] : 0.f;
+            const int kb = idx & (BK - 1), c = idx >> 5;          // B: k fastest (coalesced)
+            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] * s : 0.f;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int idx = it * 256 + tid;
+            As[buf][idx & (MBM - 1)][idx >> 7] = to_bf16(ra[it]);
+            Bs[buf][idx >> 5][idx & (BK - 1)] = to_bf16(rb[it]);
+        }
+    };
+    const int nk = (K + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int st = 0; st < nk; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nk) gload((st + 1) * BK);
+        const int kq = 8 * (lane >> 4);
+        v8bf x[4], y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = *reinterpret_cast<const v8bf*>(&Bs[buf][wn * 64 + i * 16 + (lane & 15)][kq]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) y[j] = *reinterpret_cast<const v8bf*>(&As[buf][wm * 64 + j * 16 + (lane & 15)][kq]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], y[j], acc[i][j], 0, 0, 0);
+        if (st + 1 < nk) {
+            sstore(buf ^ 1);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
+                if (row < M && col < N) {
+                    float* cp = Cg + (int64_t)col * ldc + row;
+                    *cp = *cp - acc[i][j][q] * inv;
+                }
+            }
+        }
+}
+
+// mode 0: R = B - A X (FP64, nrhs <= TRHS columns) and, per column, partial max |r| and max |x|
+//         of this workgroup's rows: part[blk * 2 TRHS + {s, TRHS + s}];
+// mode 1: partial max_i sum_k |A[i,k]| (the inf-norm of A) into part[blk];
+// mode 2: R = A X.
+constexpr int RRB = 64;
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict A, int64_t ld, int64_t n,
+                                                  const double* __restrict B, int64_t ldb, const double* __restrict X,
+                                                  int64_t ldx, double* __restrict R, int64_t ldr, int nrhs,
+                                                  double* __restrict part, int mode) {
+    const int rowsum = mode == 1;
+    const int64_t r0 = (int64_t)blockIdx.x * RRB;
+    const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
+    const int64_t row = r0 + rr;
+    __shared__ double Xs[256][TRHS + 1];
+    __shared__ double Ps[4][RRB][TRHS + 1];
+    double acc[TRHS];
+#pragma unroll
+    for (int s = 0; s < TRHS; ++s) acc[s] = 0.0;
+    for (int64_t k0 = 0; k0 < n; k0 += 256) {
+        const int kc = (int)std::min<int64_t>(256, n - k0);
+        if (!rowsum) {
+            for (int idx = tid; idx < 256 * TRHS; idx += 256) {
+                const int kk = idx & 255, s = idx >> 8;
+                Xs[kk][s] = (kk < kc && s < nrhs) ? X[(int64_t)s * ldx + k0 + kk] : 0.0;
+            }
+        }
+        __syncthreads();
+        if (row < n) {
+            for (int kk = g; kk < kc; kk += 4) {
+                const double a = A[(k0 + kk) * ld + row];
+                if (rowsum) {
+                    acc[0] += fabs(a);
+                } else {
+#pragma unroll
+                    for (int s = 0; s < TRHS; ++s) acc[s] = fma(a, Xs[kk][s], acc[s]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < TRHS; ++s) Ps[g][rr][s] = acc[s];
+    __syncthreads();
+    if (rowsum) {
+        if (tid == 0) {
+            double m = 0.0;
+            for (int i = 0; i < RRB; ++i)
+                if (r0 + i < n) m = fmax(m, Ps[0][i][0] + Ps[1][i][0] + Ps[2][i][0] + Ps[3][i][0]);
+            part[blockIdx.x] = m;
+        }
+        return;
+    }
+    __shared__ double mr[RRB][TRHS + 1];
+    __shared__ double mx[RRB][TRHS + 1];
+    for (int idx = tid; idx < RRB * TRHS; idx += 256) {
+        const int i = idx & (RRB - 1), s = idx >> 6;
+        const int64_t gr = r0 + i;
+        double rv = 0.0, xv = 0.0;
+        if (gr < n && s < nrhs) {
+            const double ax = Ps[0][i][s] + Ps[1][i][s] + Ps[2][i][s] + Ps[3][i][s];
+            rv = (mode == 0) ? B[(int64_t)s * ldb + gr] - ax : ax;
+            R[(int64_t)s * ldr + gr] = rv;
+            xv = X[(int64_t)s * ldx + gr];
+        }
+        mr[i][s] = fabs(rv);
+        mx[i][s] = fabs(xv);
+    }
+    if (mode == 2) return;
+    __syncthreads();
+    if (tid < TRHS) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < RRB; ++i) {
+            a = fmax(a, mr[i][tid]);
+            b = fmax(b, mx[i][tid]);
+        }
+        part[(int64_t)blockIdx.x * 2 * TRHS + tid] = a;
+        part[(int64_t)blockIdx.x * 2 * TRHS + TRHS + tid] = b;
+    }
+}
+
+// Y += alpha X, column-major n x nrhs.
+__global__ void k_axpy_cols(double* __restrict Y, int64_t ldy, const double* __restrict X, int64_t ldx, int64_t n,
+                            int nrhs, double alpha) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int s = 0; s < nrhs; ++s) Y[s * ldy + i] += alpha * X[s * ldx + i];
+}
+
+// Partial dot products of columns 0..k-1 of V (n x k, ld) with w, per 256-row block:
+// part[blk * k + j].  Deterministic: the host sums the partials in block order.
+__global__ __launch_bounds__(256) void k_vtw(const double* __restrict V, int64_t ld, int k, const double* __restrict w,
+                                             int64_t n, double* __restrict part) {
+    __shared__ double red[4][64];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const double wi = i < n ? w[i] : 0.0;
+    for (int j = 0; j < k; ++j) {
+        double v = i < n ? V[j * ld + i] * wi : 0.0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][j & 63] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * k + j] = red[0][j & 63] + red[1][j & 63] + red[2][j & 63] + red[3][j & 63];
+        __syncthreads();
+    }
+}
+
+// w -= V h (n x k)
+__global__ void k_wmvh(double* __restrict w, const double* __restrict V, int64_t ld, int k, const double* __restrict h,
+                       int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = w[i];
+    for (int j = 0; j < k; ++j) s -= V[j * ld + i] * h[j];
+    w[i] = s;
+}
+
+__global__ void k_scale_copy(double* __restrict dst, const double* __restrict src, double a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = a * src[i];
+}
+
+// ---------------------------------------------------------------------------------------- host
+int mixed_alloc(lsx_lu* lu) {
+    lsx_ctx* ctx = lu->ctx;
+    const int64_t n = lu->n;
+    MixedState* m = new (std::nothrow) MixedState();
+    if (!m) return LSX_ENOMEM;
+    lu->aux = m;
+    lu->ld32 = pad_ld(n);
+    const int64_t nblk = cdiv(n, DB);
+    if (hipMalloc(&lu->F32, sizeof(float) * lu->ld32 * n) != hipSuccess)
+        return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate the FP32 factors");
+    LSX_HIP(ctx, hipMalloc(&m->DL, sizeof(float) * DB * DB * nblk));
+    LSX_HIP(ctx, hipMalloc(&m->DU, sizeof(float) * DB * DB * nblk));
+    LSX_HIP(ctx, hipMalloc(&m->anorm, sizeof(double)));
+    LSX_HIP(ctx, hipMalloc(&m->scale, sizeof(float) * 8));
+    const int64_t nb = cdiv(n, RRB);
+    LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64)));
+    LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64),
+                               hipHostMallocDefault));
+    return LSX_OK;
+}
+
+void mixed_free(lsx_lu* lu) {
+    MixedState* m = ms(lu);
+    if (lu->stream) (void)hipStreamSynchronize(lu->stream);
+    (void)hipFree(lu->F32);
+    lu->F32 = nullptr;
+    if (!m) return;
+    (void)hipFree(m->DL);
+    (void)hipFree(m->DU);
+    (void)hipFree(m->anorm);
+    (void)hipFree(m->scale);
+    (void)hipFree(m->r);
+    (void)hipFree(m->xw);
+    (void)hipFree(m->bw);
+    (void)hipFree(m->part);
+    if (m->hpart) (void)hipHostFree(m->hpart);
+    (void)hipFree(m->V);
+    (void)hipFree(m->w);
+    (void)hipFree(m->hvec);
+    delete m;
+    lu->aux = nullptr;
+}
+
+static void gemm_f32(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                     int64_t M, int64_t N, int64_t K, int mode) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
+    const int pi = lu->prof.begin(lu->stream, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->stream, A, lda, B, ldb, C, ldc,
+                       (int)M, (int)N, (int)K, mode);
+    lu->prof.end(lu->stream, pi);
+}
+
+static void gemm_bf16(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                      int64_t M, int64_t N, int64_t K) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    float* sc = ms(lu)->scale;
+    hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, lu->stream, B, ldb, (int)K, (int)N, sc);
+    const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
+    const int pi = lu->prof.begin(lu->stream, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->stream, A, lda, B, ldb, C,
+                       ldc, (int)M, (int)N, (int)K, (const float*)sc);
+    lu->prof.end(lu->stream, pi);
+}
+
+int mixed_getrf(lsx_lu* lu) {
+    lsx_ctx* ctx = lu->ctx;
+    MixedState* m = ms(lu);
+    const int64_t n = lu->n;
+    // ||A||_inf in FP64 (for the backward error), then the working copy in FP32
+    const int64_t nb = cdiv(n, RRB);
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)nb), dim3(256), 0, lu->stream, lu->F, lu->ld, n, (const double*)nullptr,
+                       (int64_t)0, (const double*)nullptr, (int64_t)0, (double*)nullptr, (int64_t)0, 1, m->part, 1);
+    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb, hipMemcpyDeviceToHost, lu->stream));
+    const int pc = lu->prof.begin(lu->stream, PK_CONVERT, 0.0, 12.0 * (double)n * n);
+    hipLaunchKernelGGL(k_cast_f32, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 64), (unsigned)n), dim3(256), 0,
+                       lu->stream, lu->F, lu->ld, lu->F32, lu->ld32, n);
+    lu->prof.end(lu->stream, pc);
+    auto gt = [lu](const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc, int64_t M,
+                   int64_t N, int64_t K, int mode) { gemm_f32(lu, A, lda, B, ldb, C, ldc, M, N, K, mode); };
+    int rc;
+    if (lu->dtype == LSX_BF16) {
+        auto gu = [lu](const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc, int64_t M,
+                       int64_t N, int64_t K, int) { gemm_bf16(lu, A, lda, B, ldb, C, ldc, M, N, K); };
+        rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gu, gt);
+    } else {
+        rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gt, gt);
+    }
+    double an = 0.0;
+    for (int64_t i = 0; i < nb; ++i) an = std::max(an, m->hpart[i]);
+    m->anorm_h = an;
+    return rc;
+}
+
+int mixed_get_factors(lsx_lu* lu, double* F, int64_t ldf) {
+    // widen the FP32 factors into the caller's FP64 buffer, one column at a time through lu->wy
+    lsx_ctx* ctx = lu->ctx;
+    std::vector<float> col(lu->n);
+    std::vector<double> colw(lu->n);
+    for (int64_t j = 0; j < lu->n; ++j) {
+        LSX_HIP(ctx, hipMemcpy(col.data(), lu->F32 + j * lu->ld32, sizeof(float) * lu->n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < lu->n; ++i) colw[i] = col[i];
+        LSX_HIP(ctx, hipMemcpy(F + j * ldf, colw.data(), sizeof(double) * lu->n, hipMemcpyDefault));
+    }
+    return LSX_OK;
+}
+
+int mixed_getrs_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx) {
+    MixedState* m = ms(lu);
+    return getrs_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, nrhs, dB, ldb, dX, ldx);
+}
+
+static int ensure_mixed_work(lsx_lu* lu, int64_t nrhs) {
+    MixedState* m = ms(lu);
+    lsx_ctx* ctx = lu->ctx;
+    if (nrhs <= m->cap) return LSX_OK;
+    (void)hipFree(m->r);
+    (void)hipFree(m->xw);
+    (void)hipFree(m->bw);
+    const int64_t cap = std::max<int64_t>(nrhs, TRHS);
+    LSX_HIP(ctx, hipMalloc(&m->r, sizeof(double) * lu->n * cap));
+    LSX_HIP(ctx, hipMalloc(&m->xw, sizeof(double) * lu->n * cap));
+    LSX_HIP(ctx, hipMalloc(&m->bw, sizeof(double) * lu->n * cap));
+    m->cap = cap;
+    return lu_ensure_work(lu, cap);
+}
+
+// r = b - A x for columns [c0, c0 + nr) and the per-column backward errors.
+static int residual(lsx_lu* lu, int64_t c0, int nr, const double* bnorm, double* be) {
+    lsx_ctx* ctx = lu->ctx;
+    MixedState* m = ms(lu);
+    const int64_t n = lu->n, nb = cdiv(n, RRB);
+    const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * n * nr, 8.0 * (double)n * n);
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)nb), dim3(256), 0, lu->stream, lu->F, lu->ld, n,
+                       (const double*)(m->bw + c0 * n), n, (const double*)(m->xw + c0 * n), n, m->r + c0 * n, n, nr,
+                       m->part, 0);
+    lu->prof.end(lu->stream, pi);
+    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb * 2 * TRHS, hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    for (int s = 0; s < nr; ++s) {
+        double rn = 0.0, xn = 0.0;
+        for (int64_t b = 0; b < nb; ++b) {
+            rn = std::max(rn, m->hpart[b * 2 * TRHS + s]);
+            xn = std::max(xn, m->hpart[b * 2 * TRHS + TRHS + s]);
+        }
+        const double den = m->anorm_h * xn + bnorm[c0 + s];
+        be[s] = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
+    }
+    return LSX_OK;
+}
+
+// Deterministic dot products of V's first k columns with w (host sums block partials in order).
+static int vtw(lsx_lu* lu, const double* V, int k, const double* w, double* h) {
+    lsx_ctx* ctx = lu->ctx;
+    MixedState* m = ms(lu);
+    const int64_t n = lu->n, nb = cdiv(n, 256);
+    hipLaunchKernelGGL(k_vtw, dim3((unsigned)nb), dim3(256), 0, lu->stream, V, n, k, w, n, m->part);
+    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb * k, hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    for (int j = 0; j < k; ++j) {
+        double s = 0.0;
+        for (int64_t b = 0; b < nb; ++b) s += m->hpart[b * k + j];
+        h[j] = s;
+    }
+    return LSX_OK;
+}
+
+// Solves A d = r (column s of m->r, result into m->r column s) by GMRES in FP64 with the
+// low-precision LU as left preconditioner and classical Gram-Schmidt applied twice.
+// Returns the number of iterations or < 0.
+static int gmres_ir_correction(lsx_lu* lu, int64_t s, int maxit, double rtol) {
+    lsx_ctx* ctx = lu->ctx;
+    MixedState* m = ms(lu);
+    const int64_t n = lu->n;
+    const int M = std::max(2, std::min(maxit, 50));
+    if (m->gm < M) {
+        (void)hipFree(m->V);
+        (void)hipFree(m->w);
+        (void)hipFree(m->hvec);
+        LSX_HIP(ctx, hipMalloc(&m->V, sizeof(double) * n * (M + 1)));
+        LSX_HIP(ctx, hipMalloc(&m->w, sizeof(double) * n));
+        LSX_HIP(ctx, hipMalloc(&m->hvec, sizeof(double) * (M + 2)));
+        m->gm = M;
+    }
+    double* r = m->r + s * n;
+    const unsigned g = (unsigned)cdiv(n, 256);
+    // z0 = M^-1 r
+    int rc = mixed_getrs_dev(lu, 1, r, n, m->V, n);
+    if (rc) return rc;
+    double beta;
+    rc = vtw(lu, m->V, 1, m->V, &beta);
+    if (rc) return rc;
+    beta = std::sqrt(beta);
+    if (beta == 0.0) {
+        LSX_HIP(ctx, hipMemsetAsync(r, 0, sizeof(double) * n, lu->stream));
+        return 0;
+    }
+    hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(256), 0, lu->stream, m->V, (const double*)m->V, 1.0 / beta, n);
+    std::vector<double> H((size_t)(M + 1) * M, 0.0), cs(M), sn(M), z(M + 1, 0.0), h(M + 1), h2(M + 1);
+    z[0] = beta;
+    int k = 0;
+    const int nbr = (int)cdiv(n, RRB);
+    for (; k < M; ++k) {
+        // w = M^-1 A v_k
+        hipLaunchKernelGGL(k_residual, dim3((unsigned)nbr), dim3(256), 0, lu->stream, lu->F, lu->ld, n,
+                           (const double*)nullptr, (int64_t)0, (const double*)(m->V + k * n), n, m->w, n, 1,
+                           m->part, 2);
+        rc = mixed_getrs_dev(lu, 1, m->w, n, m->w, n);
+        if (rc) return rc;
+        // CGS2
+        rc = vtw(lu, m->V, k + 1, m->w, h.data());
+        if (rc) return rc;
+        LSX_HIP(ctx, hipMemcpyAsync(m->hvec, h.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, lu->stream));
+        hipLaunchKernelGGL(k_wmvh, dim3(g), dim3(256), 0, lu->stream, m->w, (const double*)m->V, n, k + 1,
+                           (const double*)m->hvec, n);
+        rc = vtw(lu, m->V, k + 1, m->w, h2.data());
+        if (rc) return rc;
+        LSX_HIP(ctx, hipMemcpyAsync(m->hvec, h2.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, lu->stream));
+        hipLaunchKernelGGL(k_wmvh, dim3(g), dim3(256), 0, lu->stream, m->w, (const double*)m->V, n, k + 1,
+                           (const double*)m->hvec, n);
+        for (int j = 0; j <= k; ++j) h[j] += h2[j];
+        double hn;
+        rc = vtw(lu, m->w, 1, m->w, &hn);
+        if (rc) return rc;
+        hn = std::sqrt(hn);
+        // Givens on column k of H
+        for (int j = 0; j < k; ++j) {
+            const double t = cs[j] * h[j] + sn[j] * h[j + 1];
+            h[j + 1] = -sn[j] * h[j] + cs[j] * h[j + 1];
+            h[j] = t;
+        }
+        const double rho = std::hypot(h[k], hn);
+        cs[k] = rho > 0 ? h[k] / rho : 1.0;
+        sn[k] = rho > 0 ? hn / rho : 0.0;
+        h[k] = rho;
+        for (int j = 0; j <= k; ++j) H[(size_t)j * M + k] = h[j];
+        z[k + 1] = -sn[k] * z[k];
+        z[k] = cs[k] * z[k];
+        if (hn > 0.0 && k + 1 < M)
+            hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(256), 0, lu->stream, m->V + (k + 1) * n, (const double*)m->w,
+                               1.0 / hn, n);
+        if (std::fabs(z[k + 1]) <= rtol * beta || hn == 0.0) {
+            ++k;
+            break;
+        }
+    }
+    // y = H^-1 z, d = V y
+    std::vector<double> y(k);
+    for (int i = k - 1; i >= 0; --i) {
+        double t = z[i];
+        for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * M + j] * y[j];
+        y[i] = H[(size_t)i * M + i] != 0.0 ? t / H[(size_t)i * M + i] : 0.0;
+    }
+    for (int i = 0; i < k; ++i) y[i] = -y[i];
+    LSX_HIP(ctx, hipMemcpyAsync(m->hvec, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(r, 0, sizeof(double) * n, lu->stream));
+    hipLaunchKernelGGL(k_wmvh, dim3(g), dim3(256), 0, lu->stream, r, (const double*)m->V, n, k, (const double*)m->hvec, n);
+    LSX_HIP(ctx, hipGetLastError());
+    return k;
+}
+
 }  // namespace lsx
+
+using namespace lsx;
+
+extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx,
+                             double tol, int32_t max_iters, int32_t mode, lsx_refine_stats* stats) {
+    if (!lu || !B || !X || nrhs < 1) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    if (!lu->factored) return fail(ctx, LSX_ESTATE, "lsx_lu_refine: no factorization");
+    if (lu->dtype == LSX_F64) return fail(ctx, LSX_EINVAL, "lsx_lu_refine: the handle holds FP64 factors; use lsx_lu_solve");
+    if (mode < 0 || mode > 2) return fail(ctx, LSX_EINVAL, "lsx_lu_refine: mode must be 0, 1 or 2");
+    DeviceGuard g(ctx);
+    MixedState* m = ms(lu);
+    const int64_t n = lu->n;
+    int rc = ensure_mixed_work(lu, nrhs);
+    if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpy2DAsync(m->bw, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    // ||b_j||_inf (host), from the residual kernel with x = 0
+    std::vector<double> bnorm(nrhs, 0.0), be(TRHS, 0.0);
+    LSX_HIP(ctx, hipMemsetAsync(m->xw, 0, sizeof(double) * n * nrhs, lu->stream));
+    int total_it = 0, outer = 0, mode_used = (mode == 1) ? 1 : 0;
+    double worst = 0.0;
+    bool all_conv = true;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += TRHS) {
+        const int nr = (int)std::min<int64_t>(TRHS, nrhs - c0);
+        std::vector<double> tmp(nr);
+        rc = residual(lu, c0, nr, bnorm.data(), tmp.data());  // r = b, be = ||b|| / ||b||
+        if (rc) return rc;
+        for (int s = 0; s < nr; ++s) {
+            double rn = 0.0;
+            const int64_t nb = cdiv(n, RRB);
+            for (int64_t b = 0; b < nb; ++b) rn = std::max(rn, m->hpart[b * 2 * TRHS + s]);
+            bnorm[c0 + s] = rn;
+        }
+        // x0 = LU \ b
+        rc = mixed_getrs_dev(lu, nr, m->bw + c0 * n, n, m->xw + c0 * n, n);
+        if (rc) return rc;
+        int use_gmres = (mode == 1);
+        double prev = INFINITY;
+        int stall = 0;
+        bool conv = false;
+        for (int it = 0; it <= max_iters; ++it) {
+            rc = residual(lu, c0, nr, bnorm.data(), be.data());
+            if (rc) return rc;
+            double worst_c = 0.0;
+            for (int s = 0; s < nr; ++s) worst_c = std::max(worst_c, be[s]);
+            ++outer;
+            if (worst_c <= tol) {
+                conv = true;
+                worst = std::max(worst, worst_c);
+                break;
+            }
+            if (it == max_iters) {
+                worst = std::max(worst, worst_c);
+                break;
+            }
+            if (mode == 2 && !use_gmres) {
+                stall = (worst_c > 0.5 * prev) ? stall + 1 : 0;
+                if (stall >= 2) {
+                    use_gmres = 1;
+                    mode_used = 1;
+                }
+            }
+            prev = worst_c;
+            if (use_gmres) {
+                for (int s = 0; s < nr; ++s) {
+                    const int k = gmres_ir_correction(lu, c0 + s, 30, 1e-6);
+                    if (k < 0) return k;
+                    total_it += k;
+                }
+                const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * nr, 24.0 * n * nr);
+                hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, m->xw + c0 * n, n,
+                                   (const double*)(m->r + c0 * n), n, n, nr, 1.0);
+                lu->prof.end(lu->stream, pi);
+            } else {
+                rc = mixed_getrs_dev(lu, nr, m->r + c0 * n, n, m->r + c0 * n, n);
+                if (rc) return rc;
+                const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * nr, 24.0 * n * nr);
+                hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, m->xw + c0 * n, n,
+                                   (const double*)(m->r + c0 * n), n, n, nr, 1.0);
+                lu->prof.end(lu->stream, pi);
+                ++total_it;
+            }
+        }
+        all_conv = all_conv && conv;
+    }
+    LSX_HIP(ctx, hipMemcpy2DAsync(X, sizeof(double) * ldx, m->xw, sizeof(double) * n, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t), hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    lu->prof.collect();
+    if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_refine: a block wait exceeded its bound");
+    if (stats) {
+        stats->mode_used = mode_used;
+        stats->iterations = total_it;
+        stats->outer = outer;
+        stats->converged = all_conv ? 1 : 0;
+        stats->backward_error = worst;
+    }
+    return LSX_OK;
+}

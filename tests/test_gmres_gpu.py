@@ -1,0 +1,138 @@
+"""SimpleGMRES on the GPU through the C-ABI, against the oracle (a restatement of
+src/simplegmres.jl) and the reference's closed-form tests.  Dot products are summed in a different
+order than the reference's BLAS, so parity is tolerance-based."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def problems(lsx):
+    from linearsolve_jl_amd import problems as P
+    return P
+
+
+def test_reference_6x6_resolve(lsx):
+    # test/Core/basictests.jl:651-684: rtol 1e-13 vs A \ b, then re-solve with b mutated / replaced /
+    # unchanged
+    nr = 6
+    Ar = np.array([[10.0 + i if i == j else 0.3 * (i + j) for j in range(1, nr + 1)] for i in range(1, nr + 1)])
+    br = np.arange(1.0, nr + 1)
+    br2 = np.array([0.3 * i + 1 for i in range(1, nr + 1)])
+    for restart in (True, False):
+        for mode in ("mutated", "replaced", "unchanged"):
+            A = lsx.CSRMatrix.from_scipy(sp.csr_matrix(Ar))
+            cache = lsx.init(lsx.LinearProblem(A, br.copy()), lsx.MI355XSimpleGMRES(restart=restart),
+                             abstol=1e-12, reltol=1e-12)
+            np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(Ar, br), rtol=1e-13)
+            if mode == "mutated":
+                cache.b[:] = br2
+                expected = br2
+            elif mode == "replaced":
+                cache.b = br2.copy()
+                expected = br2
+            else:
+                expected = br
+            np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(Ar, expected), rtol=1e-13)
+
+
+def test_reference_6x6_preconditioned(lsx):
+    # test/Core/basictests.jl:686-710: Pl, Pr, both; success and rtol 1e-8
+    npr = 6
+    Ap = np.array([[10.0 + 5 * i if i == j else 0.3 * (i + j) for j in range(1, npr + 1)] for i in range(1, npr + 1)])
+    bp = np.arange(1.0, npr + 1)
+    d = np.diag(Ap)
+    for restart in (True, False):
+        for Pl, Pr in ((d, None), (None, d), (np.sqrt(d), np.sqrt(d))):
+            A = lsx.CSRMatrix.from_scipy(sp.csr_matrix(Ap))
+            sol = lsx.solve(lsx.LinearProblem(A, bp.copy()), lsx.MI355XSimpleGMRES(restart=restart), Pl=Pl, Pr=Pr,
+                            abstol=1e-12, reltol=1e-12, maxiters=100)
+            assert sol.retcode == lsx.ReturnCode.Success
+            np.testing.assert_allclose(sol.u, np.linalg.solve(Ap, bp), rtol=1e-8)
+            ref = O.gmres(A.rowptr, A.colind, A.vals, bp, memory=20, restart=restart, abstol=1e-12, reltol=1e-12,
+                          maxiters=100, pl=Pl, pr=Pr)
+            assert ref["retcode"] == int(sol.retcode)
+            assert ref["iters"] == sol.iters
+
+
+@pytest.mark.parametrize("m,pe", [(8, 0.0), (16, 1.0), (32, 5.0), (64, 3.0), (100, 7.0), (128, 2.0)])
+def test_convdiff_matches_oracle(lsx, m, pe):
+    print(f"gmres convdiff m={m} pe={pe}", flush=True)
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(m, pe)
+    N = m * m
+    b = P.rng_for(7, m).standard_normal(N)
+    d = np.full(N, 4.0)
+    maxit = 3000
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XSimpleGMRES(restart=True, memory=30), Pl=d, reltol=1e-8,
+                    maxiters=maxit)
+    ref = O.gmres(rp, ci, v, b, memory=30, restart=True, reltol=1e-8, maxiters=maxit, pl=d)
+    assert int(sol.retcode) == ref["retcode"]
+    assert abs(sol.iters - ref["iters"]) <= 1
+    S = sp.csr_matrix((v, ci, rp), shape=(N, N))
+    if ref["retcode"] == O.RET_SUCCESS:
+        # converged: true preconditioned residual within the tolerance the solver stopped at
+        eps_ = np.sqrt(np.finfo(float).eps) + 1e-8 * np.linalg.norm(b / d)
+        assert np.linalg.norm((b - S @ sol.u) / d) <= 1.5 * eps_
+        np.testing.assert_allclose(sol.u, ref["u"], rtol=0, atol=1e-5 * np.abs(ref["u"]).max())
+    else:
+        np.testing.assert_allclose(sol.resid, ref["resid"], rtol=1e-3)
+
+
+@pytest.mark.parametrize("m", [256, 1024])
+def test_large_grid_residual_estimate_and_determinism(lsx, m):
+    # Full-size property (grid side up to 1024, N = 1,048,576): the Givens estimate rNorm equals the
+    # true preconditioned residual ||Pl \ (b - A x)|| (exact in exact arithmetic), and two solves are
+    # bitwise identical.
+    print(f"gmres large m={m}", flush=True)
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(m, 4.0)
+    N = m * m
+    b = P.rng_for(8, m).standard_normal(N)
+    d = np.full(N, 4.0)
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    alg = lsx.MI355XSimpleGMRES(restart=True, memory=30)
+    s1 = lsx.solve(lsx.LinearProblem(A, b), alg, Pl=d, reltol=1e-8, maxiters=120)
+    s2 = lsx.solve(lsx.LinearProblem(A, b), alg, Pl=d, reltol=1e-8, maxiters=120)
+    assert s1.iters == 120 and s1.retcode == lsx.ReturnCode.MaxIters
+    np.testing.assert_array_equal(s1.u, s2.u)
+    S = sp.csr_matrix((v, ci, rp), shape=(N, N))
+    true = np.linalg.norm((b - S @ s1.u) / d)
+    assert abs(true - s1.resid) <= 1e-6 * np.linalg.norm(b / d)
+    assert s1.resid < np.linalg.norm(b / d)
+
+
+def test_spmv_matches_scipy(lsx):
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(50, 3.0)
+    N = 2500
+    x = np.random.default_rng(1).standard_normal(N)
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    h = lsx._CSRHandle(lsx.Context.get(0), A)
+    y = h.spmv(x)
+    np.testing.assert_allclose(y, sp.csr_matrix((v, ci, rp), shape=(N, N)) @ x, rtol=1e-14, atol=1e-14)
+
+
+def test_zero_rhs_and_maxiters(lsx):
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(20, 1.0)
+    N = 400
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    sol = lsx.solve(lsx.LinearProblem(A, np.zeros(N)), lsx.MI355XSimpleGMRES())
+    assert sol.retcode == lsx.ReturnCode.Success and sol.iters == 0 and not sol.u.any()
+    b = np.ones(N)
+    sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XSimpleGMRES(memory=5), reltol=1e-14, maxiters=7)
+    ref = O.gmres(rp, ci, v, b, memory=5, restart=True, reltol=1e-14, maxiters=7)
+    assert sol.retcode == lsx.ReturnCode.MaxIters and sol.iters == 7 == ref["iters"]
+    np.testing.assert_allclose(sol.u, ref["u"], rtol=1e-10, atol=1e-12)
+
+
+def test_matrix_rhs_rejected(lsx):
+    # src/common.jl:614-625
+    A = lsx.CSRMatrix.from_scipy(sp.identity(4, format="csr"))
+    with pytest.raises(ValueError):
+        lsx.init(lsx.LinearProblem(A, np.ones((4, 2))), lsx.MI355XSimpleGMRES())

@@ -45,6 +45,11 @@ def test_getrf_matches_oracle_and_lapack(lsx, n):
     assert backward_error(A, X, B) <= 1e-13
     Xo = O.dgetrs(Fo, po, B)
     assert np.abs(X - Xo).max() <= 1e-9 * max(1.0, np.abs(Xo).max())
+    if n in (33, 300, 1025):
+        B2 = rng.standard_normal((n, 37))
+        X2 = np.empty((n, 37), order="F")
+        h.solve(B2, X2)
+        assert backward_error(A, X2, B2) <= 1e-13
 
 
 def test_permutation_matrix_exact(lsx):

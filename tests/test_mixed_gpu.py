@@ -1,0 +1,82 @@
+"""Mixed-precision LU (FP32 / BF16 factors) with FP64 iterative refinement on the GPU.
+
+Bar: normwise backward error max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+<= 1e-13, the FP64 accuracy the reference's LUFactorization delivers; the reference's own mixed
+algorithms (OpenBLAS32MixedLUFactorization, src/openblas.jl:461-543) stop at ~1e-5
+(test/Core/test_mixed_precision.jl:29-31).
+"""
+import numpy as np
+import pytest
+
+from conftest import backward_error
+
+pytestmark = pytest.mark.gpu
+
+
+def conditioned(n, cond, seed):
+    from linearsolve_jl_amd import problems as P
+    return P.conditioned(n, cond, seed=seed)
+
+
+@pytest.mark.parametrize("precision,refinement", [("fp32", "ir"), ("fp32", "auto"), ("bf16", "auto"),
+                                                  ("bf16", "gmres-ir")])
+@pytest.mark.parametrize("n", [100, 700, 1500])
+def test_random_reaches_fp64(lsx, precision, refinement, n):
+    print(f"mixed {precision} {refinement} n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 3))
+    alg = lsx.MI355XMixedLUFactorization(precision=precision, refinement=refinement, tol=1e-13, max_iters=60)
+    sol = lsx.solve(lsx.LinearProblem(A, B), alg)
+    assert sol.retcode == lsx.ReturnCode.Success, sol.stats
+    assert backward_error(A, sol.u, B) <= 1e-13
+    assert sol.stats["backward_error"] <= 1e-13
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
+def test_conditioned(lsx, cond):
+    n = 768
+    A = conditioned(n, cond, seed=1)
+    b = np.random.default_rng(2).standard_normal(n)
+    for precision in ("fp32", "bf16"):
+        alg = lsx.MI355XMixedLUFactorization(precision=precision, refinement="auto", tol=1e-13, max_iters=80)
+        sol = lsx.solve(lsx.LinearProblem(A, b), alg)
+        print(f"cond={cond:.0e} {precision}: {sol.stats}", flush=True)
+        assert sol.retcode == lsx.ReturnCode.Success
+        assert backward_error(A, sol.u, b) <= 1e-13
+        if cond <= 1e4:
+            np.testing.assert_allclose(sol.u, np.linalg.solve(A, b), rtol=0, atol=1e-9 * cond * np.abs(sol.u).max())
+
+
+def test_reference_mixed_cases(lsx):
+    # test/Core/test_mixed_precision.jl: rand(100,100) + 5I, b = rand(100)
+    rng = np.random.default_rng(0)
+    A = rng.random((100, 100)) + 5 * np.eye(100)
+    b = rng.random(100)
+    sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XMixedLUFactorization())
+    x64 = np.linalg.solve(A, b)
+    assert np.linalg.norm(sol.u - x64) / np.linalg.norm(x64) < 1e-12
+    # test/Core/resolve.jl:78-107 known answer
+    A0 = np.array([[1.0, 2.0], [3.0, 4.0]])
+    sol = lsx.solve(lsx.LinearProblem(A0.T @ A0, np.array([1.0, 2.0])), lsx.MI355XMixedLUFactorization())
+    np.testing.assert_allclose(sol.u, [-2.0, 1.5], rtol=1e-12)
+
+
+def test_singular_is_failure(lsx):
+    A = np.ones((2, 2))
+    sol = lsx.solve(lsx.LinearProblem(A, np.ones(2)), lsx.MI355XMixedLUFactorization())
+    assert sol.retcode == lsx.ReturnCode.Failure
+
+
+def test_pivots_match_lapack_fp32(lsx):
+    # FP32 factors of a matrix with decisive pivot margins pick LAPACK's pivots
+    import scipy.linalg.lapack as lapack
+    rng = np.random.default_rng(5)
+    n = 300
+    mags = rng.permutation(2.0 ** np.arange(1, n + 1) / 2.0 ** (n / 2))
+    A = np.diag(mags) @ (np.eye(n) + 0.01 * rng.standard_normal((n, n)))
+    A = A[rng.permutation(n), :]
+    h = lsx._LUHandle(lsx.Context.get(0), n, lsx.LSX_F32)
+    assert h.factor(A) == 0
+    _, ipiv = h.factors()
+    np.testing.assert_array_equal(ipiv, lapack.dgetrf(A)[1] + 1)
