@@ -110,11 +110,13 @@ typedef struct {
     int32_t outer;          /* refinement sweeps */
     int32_t converged;      /* 1 if the backward error reached tol */
     double backward_error;  /* max over columns of ||b-Ax||_inf/(||A||_inf ||x||_inf+||b||_inf) */
+    int32_t escalated;      /* 1 if mode 2 refactored in FP32 after BF16 refinement stalled */
+    int32_t pad;
 } lsx_refine_stats;
 
-/* Mixed-precision solve of the factored lu (LSX_F32/LSX_BF16, or LSX_F64) with FP64 refinement
- * to a normwise backward error <= tol.  mode 0 = IR, 1 = GMRES-IR, 2 = IR then GMRES-IR if IR
- * stalls.  A (host or device) is the original FP64 matrix when it is not already held by the
+/* Mixed-precision solve of the factored lu (LSX_F32/LSX_BF16) with FP64 refinement to a normwise
+ * backward error <= tol.  mode 0 = IR, 1 = GMRES-IR, 2 = IR, then GMRES-IR if IR stalls, then
+ * (BF16 factors only) a refactorization in FP32 if GMRES-IR stalls too.  A (host or device) is the original FP64 matrix when it is not already held by the
  * handle (pass NULL after lsx_lu_factor: the handle keeps its FP64 copy).  No reference
  * counterpart does refinement: the API shape follows OpenBLAS32MixedLUFactorization
  * (src/openblas.jl:461-543), whose FP32 result is refined here to FP64 accuracy. */
@@ -162,17 +164,26 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
 typedef struct {
     int32_t kind;          /* 0 = dense LU solve, 1 = SimpleGMRES */
     int32_t dtype;         /* dense: LSX_F64 */
-    int64_t n;             /* dense order or sparse N */
-    const double* A;       /* dense: n x n column-major (lda = n) */
-    const double* b;       /* length n */
-    int64_t nnz;           /* sparse */
-    const int32_t* rowptr; /* sparse CSR */
+    int64_t n;             /* dense order, or sparse N */
+    const double* A;       /* dense: n x n column-major (lda = n), host or device */
+    const double* b;       /* length n, host or device */
+    int64_t nnz;           /* sparse CSR (int32 indices), host or device */
+    const int32_t* rowptr;
     const int32_t* colind;
     const double* vals;
-    const double* pl_diag; /* sparse: Jacobi diagonal or NULL */
+    const double* pl_diag; /* sparse: diagonal left preconditioner or NULL */
     int32_t memory, restart;
     double abstol, reltol;
     int64_t maxiters;
+    /* synth != 0: the inputs are generated on the device instead of read from A/b/CSR:
+     *   dense:  A = I - g J, J iid N(0,1)/sqrt(n), g ~ U[1e-3, 1e-1], b iid N(0,1) from seed;
+     *   sparse: 2D 5-point convection-diffusion on a grid_m x grid_m grid with Peclet pe
+     *           (diagonal 4, west/south -1 - pe h/2, east/north -1 + pe h/2, h = 1/(grid_m+1)),
+     *           b iid N(0,1) from seed, Jacobi preconditioner (pl_diag ignored). */
+    int32_t synth;
+    int32_t grid_m;
+    uint64_t seed;
+    double pe;
 } lsx_request;
 
 typedef struct {
@@ -195,6 +206,9 @@ int lsx_serve_submit(lsx_serve* s, const lsx_request* req, double* x_out, uint64
 int lsx_serve_poll(lsx_serve* s, uint64_t ticket, lsx_result* res);
 int lsx_serve_drain(lsx_serve* s);
 double lsx_serve_clock(lsx_serve* s);
+/* Serving counters: [0] submitted, [1] completed, [2] batched launches, [3] requests served by
+ * batched launches, [4] peak bytes admitted, [5] requests waiting for admission now. */
+int lsx_serve_stats(lsx_serve* s, int64_t* out6);
 
 #ifdef __cplusplus
 }

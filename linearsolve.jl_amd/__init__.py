@@ -311,7 +311,7 @@ class MI355XMixedLUFactorization(_DenseAlg):
         rc = ReturnCode.Success if st.converged else ReturnCode.MaxIters
         return LinearSolution(cache.u, st.backward_error, rc, st.iterations, cache,
                               stats=dict(mode_used=st.mode_used, iterations=st.iterations, outer=st.outer,
-                                         backward_error=st.backward_error))
+                                         backward_error=st.backward_error, escalated=st.escalated))
 
 
 def _np(x):
@@ -586,6 +586,51 @@ class Server:
                        "lsx_serve_submit")
         self._keep[t.value] = (A, bb, x, pld)
         return t.value
+
+    def submit_dense_synth(self, n, seed):
+        """Dense request whose A = I - g J and b are generated on the device from seed."""
+        x = np.empty(int(n))
+        req = _abi.Request()
+        req.kind = 0
+        req.dtype = LSX_F64
+        req.n = int(n)
+        req.synth = 1
+        req.seed = int(seed)
+        t = ctypes.c_uint64()
+        self.ctx.check(_abi.lib().lsx_serve_submit(self.h, ctypes.byref(req), x.ctypes.data, ctypes.byref(t)),
+                       "lsx_serve_submit")
+        self._keep[t.value] = (None, None, x)
+        return t.value
+
+    def submit_gmres_synth(self, m, pe, seed, memory=30, restart=True, abstol=None, reltol=1e-8, maxiters=None):
+        """SimpleGMRES request on the m x m convection-diffusion grid generated on the device, with
+        the Jacobi preconditioner."""
+        N = int(m) * int(m)
+        x = np.empty(N)
+        req = _abi.Request()
+        req.kind = 1
+        req.dtype = LSX_F64
+        req.n = N
+        req.synth = 1
+        req.grid_m = int(m)
+        req.pe = float(pe)
+        req.seed = int(seed)
+        req.memory = memory
+        req.restart = int(bool(restart))
+        req.abstol = default_tol() if abstol is None else abstol
+        req.reltol = reltol
+        req.maxiters = N if maxiters is None else int(maxiters)
+        t = ctypes.c_uint64()
+        self.ctx.check(_abi.lib().lsx_serve_submit(self.h, ctypes.byref(req), x.ctypes.data, ctypes.byref(t)),
+                       "lsx_serve_submit")
+        self._keep[t.value] = (None, None, x)
+        return t.value
+
+    def stats(self):
+        out = np.zeros(6, dtype=np.int64)
+        self.ctx.check(_abi.lib().lsx_serve_stats(self.h, out.ctypes.data), "lsx_serve_stats")
+        keys = ("submitted", "completed", "batched_launches", "batched_requests", "peak_bytes", "waiting")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def poll(self, ticket):
         r = _abi.Result()

@@ -32,7 +32,8 @@ class LSXError(RuntimeError):
 
 class RefineStats(ctypes.Structure):
     _fields_ = [("mode_used", ctypes.c_int32), ("iterations", ctypes.c_int32), ("outer", ctypes.c_int32),
-                ("converged", ctypes.c_int32), ("backward_error", ctypes.c_double)]
+                ("converged", ctypes.c_int32), ("backward_error", ctypes.c_double), ("escalated", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class Profile(ctypes.Structure):
@@ -53,7 +54,8 @@ class Request(ctypes.Structure):
                 ("A", ctypes.c_void_p), ("b", ctypes.c_void_p), ("nnz", ctypes.c_int64),
                 ("rowptr", ctypes.c_void_p), ("colind", ctypes.c_void_p), ("vals", ctypes.c_void_p),
                 ("pl_diag", ctypes.c_void_p), ("memory", ctypes.c_int32), ("restart", ctypes.c_int32),
-                ("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxiters", ctypes.c_int64)]
+                ("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxiters", ctypes.c_int64),
+                ("synth", ctypes.c_int32), ("grid_m", ctypes.c_int32), ("seed", ctypes.c_uint64), ("pe", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -96,6 +98,7 @@ SIGNATURES = [
     ("lsx_serve_poll", ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(Result)]),
     ("lsx_serve_drain", ctypes.c_int, [_P]),
     ("lsx_serve_clock", ctypes.c_double, [_P]),
+    ("lsx_serve_stats", ctypes.c_int, [_P, _P]),
 ]
 
 _lib = None

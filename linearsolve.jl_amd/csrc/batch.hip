@@ -238,6 +238,29 @@ __global__ __launch_bounds__(BT) void k_batch_lu(BatchArgs a) {
 
 __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
+int batch_launch_raw(lsx_ctx* ctx, hipStream_t st, int64_t count, const int32_t* order, const int32_t* n,
+                     const int64_t* aoff, const int64_t* voff, double* A, const double* b, double* x, int64_t* ipiv,
+                     int32_t* perm, int64_t* info, uint32_t* counter) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    BatchArgs a;
+    a.count = count;
+    a.order = order;
+    a.n = n;
+    a.aoff = aoff;
+    a.voff = voff;
+    a.A = A;
+    a.b = b;
+    a.x = x;
+    a.ipiv = ipiv;
+    a.perm = perm;
+    a.info = info;
+    a.counter = counter;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)ctx->num_cus * 2));
+    hipLaunchKernelGGL(k_batch_lu, dim3(grid), dim3(BT), 0, st, a);
+    LSX_HIP(ctx, hipGetLastError());
+    return LSX_OK;
+}
+
 int batch_run(lsx_batch* B, const double* dA_src, const double* db, double* dx, bool copy_a) {
     lsx_ctx* ctx = B->ctx;
     if (copy_a)

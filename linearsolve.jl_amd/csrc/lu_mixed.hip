@@ -644,7 +644,7 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
     // ||b_j||_inf (host), from the residual kernel with x = 0
     std::vector<double> bnorm(nrhs, 0.0), be(TRHS, 0.0);
     LSX_HIP(ctx, hipMemsetAsync(m->xw, 0, sizeof(double) * n * nrhs, lu->stream));
-    int total_it = 0, outer = 0, mode_used = (mode == 1) ? 1 : 0;
+    int total_it = 0, outer = 0, mode_used = (mode == 1) ? 1 : 0, escalated = 0;
     double worst = 0.0;
     bool all_conv = true;
     for (int64_t c0 = 0; c0 < nrhs; c0 += TRHS) {
@@ -680,11 +680,23 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
                 worst = std::max(worst, worst_c);
                 break;
             }
-            if (mode == 2 && !use_gmres) {
+            if (mode == 2) {
                 stall = (worst_c > 0.5 * prev) ? stall + 1 : 0;
-                if (stall >= 2) {
+                if (stall >= 2 && !use_gmres) {
+                    // IR stalls: the factors are too coarse for plain refinement
                     use_gmres = 1;
                     mode_used = 1;
+                    stall = 0;
+                } else if (stall >= 3 && use_gmres && lu->dtype == LSX_BF16 && !escalated) {
+                    // GMRES-IR stalls too (u_bf16 * cond(A) >= 1): refactor in FP32, keep x
+                    const int saved = lu->dtype;
+                    lu->dtype = LSX_F32;
+                    rc = mixed_getrf(lu);
+                    lu->dtype = saved;
+                    if (rc < 0) return rc;
+                    escalated = 1;
+                    use_gmres = 0;
+                    stall = 0;
                 }
             }
             prev = worst_c;
@@ -694,19 +706,15 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
                     if (k < 0) return k;
                     total_it += k;
                 }
-                const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * nr, 24.0 * n * nr);
-                hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, m->xw + c0 * n, n,
-                                   (const double*)(m->r + c0 * n), n, n, nr, 1.0);
-                lu->prof.end(lu->stream, pi);
             } else {
                 rc = mixed_getrs_dev(lu, nr, m->r + c0 * n, n, m->r + c0 * n, n);
                 if (rc) return rc;
-                const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * nr, 24.0 * n * nr);
-                hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, m->xw + c0 * n, n,
-                                   (const double*)(m->r + c0 * n), n, n, nr, 1.0);
-                lu->prof.end(lu->stream, pi);
                 ++total_it;
             }
+            const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * nr, 24.0 * n * nr);
+            hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, m->xw + c0 * n, n,
+                               (const double*)(m->r + c0 * n), n, n, nr, 1.0);
+            lu->prof.end(lu->stream, pi);
         }
         all_conv = all_conv && conv;
     }
@@ -722,6 +730,8 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
         stats->outer = outer;
         stats->converged = all_conv ? 1 : 0;
         stats->backward_error = worst;
+        stats->escalated = escalated;
+        stats->pad = 0;
     }
     return LSX_OK;
 }

@@ -35,17 +35,22 @@ def test_random_reaches_fp64(lsx, precision, refinement, n):
 
 @pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
 def test_conditioned(lsx, cond):
+    # A = Q1 diag(s) Q2^T.  FP32 factors refine to FP64 by IR up to cond 1e6 (u32 * cond < 1).
+    # BF16 factors (u = 2^-8) precondition GMRES-IR well only while u * cond < 1; beyond that the
+    # 'auto' mode escalates to FP32 factors, and must still reach 1e-13.
     n = 768
     A = conditioned(n, cond, seed=1)
     b = np.random.default_rng(2).standard_normal(n)
     for precision in ("fp32", "bf16"):
-        alg = lsx.MI355XMixedLUFactorization(precision=precision, refinement="auto", tol=1e-13, max_iters=80)
+        alg = lsx.MI355XMixedLUFactorization(precision=precision, refinement="auto", tol=1e-13, max_iters=60)
         sol = lsx.solve(lsx.LinearProblem(A, b), alg)
         print(f"cond={cond:.0e} {precision}: {sol.stats}", flush=True)
         assert sol.retcode == lsx.ReturnCode.Success
         assert backward_error(A, sol.u, b) <= 1e-13
         if cond <= 1e4:
             np.testing.assert_allclose(sol.u, np.linalg.solve(A, b), rtol=0, atol=1e-9 * cond * np.abs(sol.u).max())
+        if precision == "bf16" and cond <= 1e2:
+            assert sol.stats["escalated"] == 0
 
 
 def test_reference_mixed_cases(lsx):
@@ -59,7 +64,8 @@ def test_reference_mixed_cases(lsx):
     # test/Core/resolve.jl:78-107 known answer
     A0 = np.array([[1.0, 2.0], [3.0, 4.0]])
     sol = lsx.solve(lsx.LinearProblem(A0.T @ A0, np.array([1.0, 2.0])), lsx.MI355XMixedLUFactorization())
-    np.testing.assert_allclose(sol.u, [-2.0, 1.5], rtol=1e-12)
+    # backward error 1e-13 on a cond ~ 1e3 matrix bounds the forward error by ~1e-10
+    np.testing.assert_allclose(sol.u, [-2.0, 1.5], rtol=1e-10)
 
 
 def test_singular_is_failure(lsx):
@@ -72,7 +78,7 @@ def test_pivots_match_lapack_fp32(lsx):
     # FP32 factors of a matrix with decisive pivot margins pick LAPACK's pivots
     import scipy.linalg.lapack as lapack
     rng = np.random.default_rng(5)
-    n = 300
+    n = 100  # magnitudes 2^-49 .. 2^50 stay inside the FP32 range
     mags = rng.permutation(2.0 ** np.arange(1, n + 1) / 2.0 ** (n / 2))
     A = np.diag(mags) @ (np.eye(n) + 0.01 * rng.standard_normal((n, n)))
     A = A[rng.permutation(n), :]
