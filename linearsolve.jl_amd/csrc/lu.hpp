@@ -6,7 +6,7 @@ namespace lsx {
 constexpr int PW = 32;        // panel width held in registers by the panel kernel
 constexpr int NBO = 128;      // outer block: K of the trailing MFMA update
 constexpr int PMAX = 128;     // max workgroups cooperating on one panel (512 rows each)
-constexpr int REC = 4 + 2 * PW;   // words of one exchange record: key lo/hi, row, pad, data
+constexpr int REC = 4 + 2 * PW;   // words of one FP64 exchange record: key lo/hi, row, pad, data
 constexpr int RECS = 72;          // record stride in the exchange buffer (words)
 constexpr int DB = 64;            // diagonal block of the explicit inverses and triangular solves
 constexpr int TRHS = 16;          // right-hand sides per triangular-solve launch

@@ -98,21 +98,30 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict Ag, i
         }
     }
     if (mode == 1) __syncthreads();  // in place on B: every wave has finished reading B
-    // lane holds C(row = rbase + (lane & 15), col = cbase + 4q + (lane >> 4))
+    // lane holds C(row = rbase + (lane & 15), col = cbase + 4q + (lane >> 4)).  Sixteen C values
+    // at a time are loaded before any of them is stored, so the loads overlap.
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        double cv[4][4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int row = row0 + wm * 64 + j * 16 + (lane & 15);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int col = col0 + wn * 64 + i * 16 + 4 * q + (lane >> 4);
-                if (row < M && col < N) {
-                    double* cp = Cg + (int64_t)col * ldc + row;
-                    *cp = (mode == 0) ? (*cp - acc[i][j][q]) : acc[i][j][q];
-                }
+                cv[j][q] = (mode == 0 && row < M && col < N) ? Cg[(int64_t)col * ldc + row] : 0.0;
             }
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = col0 + wn * 64 + i * 16 + 4 * q + (lane >> 4);
+                if (row < M && col < N) Cg[(int64_t)col * ldc + row] = cv[j][q] - acc[i][j][q] * (mode == 0 ? 1.0 : -1.0);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 // LAPACK dgetf2).
 #pragma once
 #include <cfloat>
+#include <climits>
 #include <limits>
 
 #include "lu.hpp"
@@ -32,40 +33,71 @@ struct PanelArgs {
     int64_t timeout;
 };
 
-__device__ __forceinline__ bool better(double ka, int64_t ra, double kb, int64_t rb) {
-    return ka > kb || (ka == kb && ra < rb);
-}
-
-// Words (32-bit) per element and flag-tagged message writes of one element.
-template <typename T> struct Words;
-template <> struct Words<double> {
-    static constexpr int W = 2;
-    __device__ static void put(uint64_t* p, double x, uint32_t seq) {
-        st_relaxed_u64(p, ll_pack((uint32_t)__double2loint(x), seq));
-        st_relaxed_u64(p + 1, ll_pack((uint32_t)__double2hiint(x), seq));
-    }
-    __device__ static double key(const uint32_t* w) { return __hiloint2double((int)w[1], (int)w[0]); }
-};
-template <> struct Words<float> {
-    static constexpr int W = 1;
-    __device__ static void put(uint64_t* p, float x, uint32_t seq) {
-        st_relaxed_u64(p, ll_pack((uint32_t)__float_as_uint(x), seq));
-    }
-    __device__ static double key(const uint32_t* w) { return (double)__uint_as_float(w[0]); }
-};
-// Exchange record (32-bit words): key at [0, WPE), row at [WPE], row data from word 4.
+// Words (32-bit) per element of an exchange record.
+template <typename T> struct Words { static constexpr int W = sizeof(T) / 4; };
+// Exchange record (32-bit words): [0,1] key (lo, hi), [2] logical row, [3] pad, [4, ...) the
+// candidate row in the panel's register layout.  16-byte multiple, so the row data is aligned.
 template <typename T> constexpr int rec_words() { return 4 + PW * Words<T>::W; }
+
+// ---- wave-wide reductions (64 lanes): DPP within rows of 16 lanes, then two cross-row steps
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+    return ((uint64_t)dpp32<CTRL>((uint32_t)(x >> 32)) << 32) | dpp32<CTRL>((uint32_t)x);
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+    uint64_t y;
+    y = dpp64<0xB1>(x);  x = y > x ? y : x;   // quad_perm [1,0,3,2]: lane ^ 1
+    y = dpp64<0x4E>(x);  x = y > x ? y : x;   // quad_perm [2,3,0,1]: lane ^ 2
+    y = dpp64<0x141>(x); x = y > x ? y : x;   // row_half_mirror: pairs the two quads of each 8
+    y = dpp64<0x140>(x); x = y > x ? y : x;   // row_mirror: pairs the two halves of each 16
+    y = __shfl_xor((unsigned long long)x, 16); x = y > x ? y : x;
+    y = __shfl_xor((unsigned long long)x, 32); x = y > x ? y : x;
+    return x;
+}
+__device__ __forceinline__ int wave_min_i32(int x) {
+    int y;
+    y = (int)dpp32<0xB1>((uint32_t)x);  x = y < x ? y : x;
+    y = (int)dpp32<0x4E>((uint32_t)x);  x = y < x ? y : x;
+    y = (int)dpp32<0x141>((uint32_t)x); x = y < x ? y : x;
+    y = (int)dpp32<0x140>((uint32_t)x); x = y < x ? y : x;
+    y = __shfl_xor(x, 16); x = y < x ? y : x;
+    y = __shfl_xor(x, 32); x = y < x ? y : x;
+    return x;
+}
+// Lane holding the best (u, lr) pair of the wave: largest u, then smallest logical row lr.
+// u = 0 marks "no candidate"; the result is uniform across the wave.
+__device__ __forceinline__ int wave_argmax(uint64_t u, int lr) {
+    const uint64_t m = wave_max_u64(u);
+    const uint64_t tie = __ballot(u == m);
+    if (__popcll(tie) == 1 || m == 0) return __ffsll((unsigned long long)tie) - 1;
+    const int mn = wave_min_i32(u == m ? lr : INT_MAX);
+    return __ffsll((unsigned long long)__ballot(u == m && lr == mn)) - 1;
+}
+// Pivot key: |a| as its (monotone) bit pattern + 1, NaN counted as 0; 0 = not a candidate.
+template <typename T> __device__ __forceinline__ uint64_t pivot_key(T x) {
+    double t = fabs((double)x);
+    if (!(t == t)) t = 0.0;
+    return (uint64_t)__double_as_longlong(t) + 1u;
+}
 
 // Each thread owns one row of the panel in registers, in a rotating layout: at column c the
 // physical slot p holds logical column c + p (p < PW - c, the not yet eliminated columns) and
-// the finished multipliers L[r, 0..c) sit in the last c slots.  Column c is therefore always
-// v[0] and every register access is static, with a compact (not unrolled) column loop.
-// All rows rotate in lockstep, so row records exchanged between threads share the layout.
+// the finished values (multipliers L[r, 0..c), or the U row of a pivot row) sit in the last c
+// slots.  Column c is therefore always v[0] and every register access is static, with a
+// compact (not unrolled) column loop.
+// Rows do not move while the panel is factored: each thread carries the logical row index lr
+// of the row it holds.  The interchange of column k relabels two threads (the pivot row's
+// becomes k, the one labelled k takes the pivot row's old label) and marks the pivot row
+// finished; at the end every row is written to its logical position.  Ties of the pivot search
+// are broken by the logical index, as an explicit row interchange would.
+// P workgroups of BS threads cover rows [j0, m); with P > 1 each column exchanges one record
+// per workgroup (its best candidate and that row's data) as flag-in-word messages.
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
     constexpr int NW = BS / 64;
-    constexpr int WPE = Words<T>::W;
-    constexpr int RECT = rec_words<T>();
+    constexpr int RW = rec_words<T>();
     const int tid = threadIdx.x, w = blockIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r = a.j0 + (int64_t)w * BS + tid;
     const bool valid = r < a.m;
@@ -73,96 +105,60 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
     T v[PW];
 #pragma unroll
     for (int q = 0; q < PW; ++q) v[q] = (valid && q < a.W) ? col0[q * a.ld + r] : T(0);
+    int lr = (int)r;
+    bool live = valid;
+    int64_t info0 = 0;
+    if (w == 0 && tid == 0) info0 = a.dinfo[0];
 
-    __shared__ double s_wkey[2][NW];
-    __shared__ int64_t s_wrow[2][NW];
-    __shared__ __align__(16) uint32_t s_rec[2][(PMAX + 1) * RECT];
+    __shared__ __align__(16) uint32_t s_cand[2][NW][RW];
+    __shared__ __align__(16) uint32_t s_rec[2][PMAX * RW];
 
     const int P = a.P;
 #pragma unroll 1
     for (int c = 0; c < a.W; ++c) {
         const int par = c & 1;
-        const int64_t k = a.j0 + c;
-        double key = -1.0;
-        if (valid && r >= k) {
-            const double t = fabs((double)v[0]);
-            key = (t == t) ? t : 0.0;  // NaN never wins (treated as 0)
-        }
-        int64_t row = r;
+        const int k = (int)(a.j0 + c);
+        // wave candidate -> LDS
+        const uint64_t u = live ? pivot_key(v[0]) : 0;
+        const int wl = wave_argmax(u, lr);
+        if (lane == wl) {
+            uint32_t* d = s_cand[par][wave];
+            *reinterpret_cast<uint64_t*>(d) = u;
+            d[2] = (uint32_t)lr;
+            T* dv = reinterpret_cast<T*>(d + 4);
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ok = __shfl_xor(key, off);
-            const int64_t orow = __shfl_xor((long long)row, off);
-            if (better(ok, orow, key, row)) { key = ok; row = orow; }
+            for (int q = 0; q < PW; ++q) dv[q] = v[q];
         }
-        if (lane == 0) { s_wkey[par][wave] = key; s_wrow[par][wave] = row; }
         __syncthreads();
-        double bkey = s_wkey[par][0];
-        int64_t brow = s_wrow[par][0];
-#pragma unroll
-        for (int i = 1; i < NW; ++i) {
-            const double kk = s_wkey[par][i];
-            const int64_t rr = s_wrow[par][i];
-            if (better(kk, rr, bkey, brow)) { bkey = kk; brow = rr; }
+        // workgroup candidate (every wave reduces the NW candidates itself)
+        int wi;
+        {
+            const uint64_t cu = lane < NW ? *reinterpret_cast<const uint64_t*>(s_cand[par][lane]) : 0;
+            const int cl = lane < NW ? (int)s_cand[par][lane][2] : INT_MAX;
+            wi = wave_argmax(cu, cl);
         }
-        uint32_t* rec = s_rec[par];
-        int pwin = 0;
-        int64_t grow = brow;
-        if (P == 1) {
-            if (valid && r == brow) {
-                T* d = reinterpret_cast<T*>(rec + 4);
-#pragma unroll
-                for (int q = 0; q < PW; ++q) d[q] = v[q];
-            }
-            if (tid == c) {
-                T* d = reinterpret_cast<T*>(rec + RECT + 4);
-#pragma unroll
-                for (int q = 0; q < PW; ++q) d[q] = v[q];
-            }
-            __syncthreads();
-        } else {
+        const uint32_t* win = s_cand[par][wi];
+        if (P > 1) {
             const uint32_t seq = a.seq + (uint32_t)c + 1u;
             uint64_t* g = a.slots + (size_t)par * (PMAX + 1) * RECS;
-            if (valid && r == brow) {
-                uint64_t* my = g + (size_t)w * RECS;
-                Words<T>::put(my, (T)bkey, seq);
-                st_relaxed_u64(my + WPE, ll_pack((uint32_t)brow, seq));
-#pragma unroll
-                for (int q = 0; q < PW; ++q) Words<T>::put(my + 4 + WPE * q, v[q], seq);
-            }
-            if (w == 0 && tid == c) {
-                uint64_t* kr = g + (size_t)P * RECS;
-#pragma unroll
-                for (int q = 0; q < PW; ++q) Words<T>::put(kr + 4 + WPE * q, v[q], seq);
-            }
-            // Every thread reads a share of the words of all P records and of the row-k record:
-            // all of its loads are issued at once, then the ones not yet current are re-polled.
-            constexpr int RW = WPE + 1 + WPE * PW;  // words read per candidate record
-            constexpr int MAXW = (PMAX * RW + 2 * PW + BS - 1) / BS;
-            const int total = P * RW + WPE * PW;
+            if (tid < RW) st_relaxed_u64(g + (size_t)w * RECS + tid, ll_pack(win[tid], seq));
+            // every thread reads a share of the words of all P records: all of its loads are
+            // issued at once, then the ones not yet current are re-polled
+            constexpr int MAXW = (PMAX * RW + BS - 1) / BS;
+            const int total = P * RW;
             uint64_t x[MAXW];
-            auto word = [&](int idx, int& dst) -> const uint64_t* {
-                int rr = P, ww = 4 + (idx - P * RW);
-                if (idx < P * RW) {
-                    rr = idx / RW;
-                    const int wi = idx - rr * RW;
-                    ww = wi <= WPE ? wi : wi - (WPE + 1) + 4;
-                }
-                dst = rr * RECT + ww;
-                return g + (size_t)rr * RECS + ww;
-            };
 #pragma unroll
             for (int i = 0; i < MAXW; ++i) {
                 const int idx = tid + i * BS;
-                int d;
-                x[i] = (idx < total) ? ld_relaxed_u64(word(idx, d)) : 0;
+                const int rr = idx / RW;
+                x[i] = (idx < total) ? ld_relaxed_u64(g + (size_t)rr * RECS + (idx - rr * RW)) : 0;
             }
 #pragma unroll
             for (int i = 0; i < MAXW; ++i) {
                 const int idx = tid + i * BS;
                 if (idx >= total) continue;
-                int d;
-                const uint64_t* src = word(idx, d);
+                const int rr = idx / RW;
+                const uint64_t* src = g + (size_t)rr * RECS + (idx - rr * RW);
                 if ((uint32_t)(x[i] >> 32) != seq) {
                     const int64_t t0 = wall_clock64();
                     while (true) {
@@ -174,43 +170,51 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
                         }
                     }
                 }
-                rec[d] = (uint32_t)x[i];
+                s_rec[par][idx] = (uint32_t)x[i];
             }
             __syncthreads();
-            double gk = Words<T>::key(rec);
-            int64_t gr = (int64_t)rec[WPE];
-            for (int p = 1; p < P; ++p) {
-                const double kk = Words<T>::key(rec + p * RECT);
-                const int64_t rr = (int64_t)rec[p * RECT + WPE];
-                if (better(kk, rr, gk, gr)) { gk = kk; gr = rr; pwin = p; }
-            }
-            grow = gr;
-        }
-        const T* prow = reinterpret_cast<const T*>(rec + pwin * RECT + 4);
-        const T* krow = reinterpret_cast<const T*>(rec + P * RECT + 4);
-        const T pv = prow[0];
-        if (grow != k) {
-            // row k takes the pivot row, the pivot row takes row k
-            const T* src = (valid && r == k) ? prow : ((valid && r == grow) ? krow : nullptr);
-            if (src) {
+            // winner among the P records: lane l looks at records l and l + 64
+            uint64_t bu = 0;
+            int bl = INT_MAX, bp = 0;
 #pragma unroll
-                for (int q = 0; q < PW; ++q) v[q] = src[q];
+            for (int h = 0; h < PMAX / 64; ++h) {
+                const int p = lane + 64 * h;
+                if (p < P) {
+                    const uint32_t* rp = s_rec[par] + p * RW;
+                    const uint64_t pu = *reinterpret_cast<const uint64_t*>(rp);
+                    const int pl = (int)rp[2];
+                    if (pu > bu || (pu == bu && pl < bl)) { bu = pu; bl = pl; bp = p; }
+                }
             }
+            const int wlane = wave_argmax(bu, bl);
+            const int pw = __shfl(bp, wlane);
+            win = s_rec[par] + pw * RW;
+        }
+        const T* prow = reinterpret_cast<const T*>(win + 4);
+        const int grow = (int)win[2];
+        const T pv = prow[0];
+        // interchange rows k and grow by relabelling
+        if (lr == grow) {
+            live = false;
+            lr = k;
+        } else if (lr == k) {
+            lr = grow;
         }
         if (w == 0 && tid == 0) {
-            a.ipiv[k] = grow + 1;
-            if (pv == T(0) && a.dinfo[0] == 0) a.dinfo[0] = k + 1;
+            a.ipiv[k] = (int64_t)grow + 1;
+            if (pv == T(0) && info0 == 0) {
+                info0 = k + 1;
+                a.dinfo[0] = info0;
+            }
         }
         // scale, rank-1 update of the remaining columns, rotate by one slot
         T l = v[0];
-        const bool upd = valid && r > k;
-        if (upd && pv != T(0)) {
+        if (live && pv != T(0)) {
             if (fabs(pv) >= std::numeric_limits<T>::min()) l *= (T(1) / pv);
             else l /= pv;
         }
-        const int lim = upd ? PW - 1 - c : 0;  // slots 1..lim hold columns still to eliminate
+        const int lim = live ? PW - 1 - c : 0;  // slots 1..lim hold columns still to eliminate
         // in groups of 8 slots so that the pivot-row LDS reads are not all hoisted at once
-        // (keeps the 1024-thread variant within 128 VGPRs)
 #pragma unroll
         for (int p0 = 0; p0 < PW; p0 += 8) {
 #pragma unroll
@@ -223,11 +227,13 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
         v[PW - 1] = l;
     }
     if (valid) {
-        // after W rotations slot p holds logical column (p + W) mod PW
+        // after W rotations slot p holds logical column (p + W) mod PW; the row goes to its
+        // logical position
+        T* dst = col0 + lr;
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
             const int q = (p + a.W) & (PW - 1);
-            if (q < a.W) col0[q * a.ld + r] = v[p];
+            if (q < a.W) dst[q * a.ld] = v[p];
         }
     }
 }
