@@ -8,6 +8,17 @@ getrf, getrs with 16 right-hand sides, inputs already resident in HBM.
 
 Prints one JSON line.  `value` = TFLOP/s of one step (luflop(n) + 2 n^2 nrhs flops, the
 reference's own flop convention, lib/LinearSolveAutotune/src/algorithms.jl:147-159).
+
+Beside it, in the same line (each can be switched off with --sections):
+  mixed    n = 16384, A = Q1 diag(s) Q2^T with s log-spaced from 1 to 1/cond, cond in
+           {1e2, 1e4, 1e6}, one right-hand side: FP32 and BF16 factorization + FP64 refinement
+           (IR, then GMRES-IR where IR stalls) to backward error <= 1e-13, against the plain FP64
+           LU of the same system;
+  batched  20,000 systems A_i = I - g_i J_i, n_i uniform in [8, 512], one right-hand side each,
+           one persistent launch, against a loop of plain per-system GPU LU solves (timed on a
+           sample of the same systems);
+  serving  Poisson arrivals of the 70 % dense / 30 % SimpleGMRES mix, p50 / p99 latency and the
+           served rate.
 """
 import argparse
 import ctypes
@@ -85,6 +96,206 @@ def cpu_baseline(A, B, n, nrhs):
                        f"LUFactorization calls; julia is not installed), {t:.2f} s"), (piv + 1 if ns == n else None)
 
 
+def backward_error_t(A, X, B):
+    """max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), in FP64 on the device."""
+    import torch
+    R = B - A @ X
+    nA = A.abs().sum(dim=1).max()
+    num = R.abs().amax(dim=0)
+    den = nA * X.abs().amax(dim=0) + B.abs().amax(dim=0)
+    return float((num / den).max())
+
+
+def timed(fn, reps):
+    import torch
+    torch.cuda.synchronize()
+    best = float("inf")
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def run_mixed(lsx, _abi, ctx, L, n, conds, reps):
+    """Mixed-precision LU + FP64 refinement against plain FP64 LU on conditioned systems."""
+    import torch
+    P = lsx.problems
+    hs = {}
+    for name, dt in (("fp64", _abi.LSX_F64), ("fp32", _abi.LSX_F32), ("bf16", _abi.LSX_BF16)):
+        h = ctypes.c_void_p()
+        ctx.check(L.lsx_lu_create(ctx.handle, n, dt, ctypes.byref(h)), "lsx_lu_create")
+        hs[name] = h
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    b = torch.randn(n, 1, dtype=torch.float64, device="cuda", generator=g)
+    x = torch.empty_like(b)
+    out = {}
+    try:
+        for cond in conds:
+            log(f"mixed: generating cond={cond:.0e} system (QR of two Gaussian matrices on the GPU)")
+            A = P.conditioned(n, cond, device="cuda")   # (n, n) tensor, row-major
+            Acm = A.t().contiguous()                      # row-major A^T = column-major A
+            torch.cuda.synchronize()
+            ent = {}
+
+            def f64():
+                ctx.check(L.lsx_lu_factor(hs["fp64"], Acm.data_ptr(), n), "lsx_lu_factor")
+                ctx.check(L.lsx_lu_solve(hs["fp64"], 1, b.data_ptr(), n, x.data_ptr(), n), "lsx_lu_solve")
+
+            t64, _ = timed(f64, reps)
+            ent["fp64"] = dict(ms=round(t64 * 1e3, 3), backward_error=backward_error_t(A, x, b))
+            for name in ("fp32", "bf16"):
+                st = _abi.RefineStats()
+
+                def mixed():
+                    ctx.check(L.lsx_lu_factor(hs[name], Acm.data_ptr(), n), "lsx_lu_factor")
+                    ctx.check(L.lsx_lu_refine(hs[name], 1, b.data_ptr(), n, x.data_ptr(), n, 1e-13, 100, 2,
+                                              ctypes.byref(st)), "lsx_lu_refine")
+
+                tm, _ = timed(mixed, reps)
+                ent[name] = dict(ms=round(tm * 1e3, 3), speedup_vs_fp64=round(t64 / tm, 3),
+                                 backward_error=backward_error_t(A, x, b),
+                                 mode=("IR", "GMRES-IR")[st.mode_used], iterations=int(st.iterations),
+                                 converged=bool(st.converged), escalated_to_fp32=bool(st.escalated))
+                log(f"  cond={cond:.0e} {name}: {tm * 1e3:.1f} ms vs fp64 {t64 * 1e3:.1f} ms "
+                    f"({t64 / tm:.2f}x), bwd {ent[name]['backward_error']:.2e}")
+            out[f"cond={cond:.0e}"] = ent
+            del A, Acm
+            torch.cuda.empty_cache()
+    finally:
+        for h in hs.values():
+            L.lsx_lu_destroy(h)
+    return out
+
+
+def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
+    """20,000 small systems in one persistent launch vs a loop of per-system plain GPU LU solves."""
+    import torch
+    P = lsx.problems
+    g = P.rng_for(4, count, 4)
+    sizes = g.integers(8, 512 + 1, size=count).astype(np.int64)
+    gam = g.uniform(1e-3, 1e-1, size=count)
+    nn = sizes * sizes
+    aoff = np.concatenate([[0], np.cumsum(nn)[:-1]])
+    voff = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    total, totv = int(nn.sum()), int(sizes.sum())
+    log(f"batched: {count} systems, {total * 8 / 2**30:.1f} GiB of matrices, generated on the GPU")
+    tg = torch.Generator(device="cuda")
+    tg.manual_seed(44)
+    A = torch.randn(total, dtype=torch.float64, device="cuda", generator=tg)
+    scale = torch.repeat_interleave(torch.from_numpy(-gam / np.sqrt(sizes)).cuda(), torch.from_numpy(nn).cuda())
+    A.mul_(scale)
+    del scale
+    didx = np.concatenate([aoff[i] + np.arange(sizes[i]) * (sizes[i] + 1) for i in range(count)])
+    A[torch.from_numpy(didx).cuda()] += 1.0
+    b = torch.randn(totv, dtype=torch.float64, device="cuda", generator=tg)
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    hb = ctypes.c_void_p()
+    ctx.check(L.lsx_batch_create(ctx.handle, count, sizes.ctypes.data, _abi.LSX_F64, ctypes.byref(hb)),
+              "lsx_batch_create")
+    info = np.zeros(count, dtype=np.int64)
+
+    def batched():
+        return ctx.check(L.lsx_batch_factor_solve(hb, A.data_ptr(), b.data_ptr(), x.data_ptr(),
+                                                  info.ctypes.data, None), "lsx_batch_factor_solve")
+
+    batched()
+    tb, nfail = timed(batched, reps)
+    L.lsx_batch_destroy(hb)
+    # backward error on a sample of systems (host FP64)
+    rs = np.random.default_rng(7)
+    pick = rs.choice(count, size=min(200, count), replace=False)
+    worst = 0.0
+    for i in pick:
+        n = int(sizes[i])
+        Ai = A[aoff[i]:aoff[i] + n * n].view(n, n).t().cpu().numpy()
+        bi = b[voff[i]:voff[i] + n].cpu().numpy()
+        xi = x[voff[i]:voff[i] + n].cpu().numpy()
+        r = bi - Ai @ xi
+        worst = max(worst, np.abs(r).max() / (np.abs(Ai).sum(axis=1).max() * np.abs(xi).max() + np.abs(bi).max()))
+    # per-system loop of the plain FP64 GPU LU (handles made beforehand, outside the timing)
+    loop_idx = rs.choice(count, size=min(loop_sample, count), replace=False)
+    handles = {}
+    for i in loop_idx:
+        n = int(sizes[i])
+        if n not in handles:
+            h = ctypes.c_void_p()
+            ctx.check(L.lsx_lu_create(ctx.handle, n, _abi.LSX_F64, ctypes.byref(h)), "lsx_lu_create")
+            handles[n] = h
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in loop_idx:
+        n = int(sizes[i])
+        h = handles[n]
+        ctx.check(L.lsx_lu_factor(h, A.data_ptr() + 8 * int(aoff[i]), n), "lsx_lu_factor")
+        ctx.check(L.lsx_lu_solve(h, 1, b.data_ptr() + 8 * int(voff[i]), n, x.data_ptr() + 8 * int(voff[i]), n),
+                  "lsx_lu_solve")
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) / len(loop_idx)
+    for h in handles.values():
+        L.lsx_lu_destroy(h)
+    flops = float(sum(luflop(int(n)) + 2.0 * int(n) * int(n) for n in sizes))
+    res = dict(systems=count, sizes="n uniform in [8, 512]", ms=round(tb * 1e3, 3),
+               solves_per_s=round(count / tb, 1), tflops=round(flops / tb / 1e12, 3), failed=int(nfail),
+               backward_error_max_sampled=worst,
+               per_system_loop=dict(solves_per_s=round(1.0 / tl, 1), sample=len(loop_idx),
+                                    kind="lsx_lu_factor + lsx_lu_solve per system, handles reused"),
+               speedup_vs_loop=round((count / tb) * tl, 2))
+    log(f"batched: {tb * 1e3:.1f} ms, {count / tb:.0f} solves/s, loop {1 / tl:.0f} solves/s, "
+        f"speedup {res['speedup_vs_loop']}x, bwd {worst:.2e}")
+    del A, b, x
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_serving(lsx, seconds, rate, gmres_maxiters, budget_gib):
+    """Poisson arrivals of the serving mix on the synthetic (device-generated) request path."""
+    P = lsx.problems
+    trace = P.serving_trace(rate=rate, seconds=seconds)
+    srv = lsx.Server(hbm_budget=int(budget_gib) << 30, nstreams=4)
+    pending = []
+    done = {}
+    c0 = srv.clock()
+    t0 = time.perf_counter()
+    for req in trace:
+        while time.perf_counter() - t0 < req["t"]:
+            pass
+        if req["kind"] == "dense":
+            t = srv.submit_dense_synth(req["n"], req["seed"])
+        else:
+            t = srv.submit_gmres_synth(req["m"], req["pe"], req["seed"], memory=30, reltol=1e-8,
+                                       maxiters=gmres_maxiters)
+        pending.append((t, req["kind"]))
+    log(f"serving: {len(trace)} requests submitted in {time.perf_counter() - t0:.1f} s; draining")
+    srv.drain()
+    t_end = srv.clock()
+    lat = {"dense": [], "gmres": []}
+    conv = 0
+    for t, kind in pending:
+        r = srv.poll(t)
+        lat[kind].append(r["t_done"] - r["t_submit"])
+        if kind == "gmres" and r["retcode"] == lsx.ReturnCode.Success:
+            conv += 1
+    st = srv.stats()
+    srv.close()
+    allv = np.array(lat["dense"] + lat["gmres"])
+    q = lambda v, p: round(float(np.percentile(v, p)) * 1e3, 3) if len(v) else None
+    res = dict(offered_req_per_s=rate, arrival_seconds=seconds, requests=len(trace),
+               served_req_per_s=round(len(trace) / (t_end - c0), 1),
+               p50_ms=q(allv, 50), p99_ms=q(allv, 99),
+               dense=dict(count=len(lat["dense"]), p50_ms=q(lat["dense"], 50), p99_ms=q(lat["dense"], 99)),
+               gmres=dict(count=len(lat["gmres"]), p50_ms=q(lat["gmres"], 50), p99_ms=q(lat["gmres"], 99),
+                          converged=conv, maxiters=gmres_maxiters),
+               batched_launches=st["batched_launches"], batched_requests=st["batched_requests"],
+               peak_hbm_gib=round(st["peak_bytes"] / 2**30, 2))
+    log(f"serving: served {res['served_req_per_s']} req/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,16 +304,25 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sections", default="mixed,batched,serving",
+                    help="comma list of extra sections beside the FP64 LU line (mixed, batched, serving; '' = none)")
+    ap.add_argument("--mixed-n", type=int, default=16384)
+    ap.add_argument("--batch-count", type=int, default=20000)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-gmres-maxiters", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the dense LU would need a 2D block-cyclic layout with per-column pivot all-reduces)",
               file=sys.stderr)
         sys.exit(2)
+    sections = [x for x in args.sections.split(",") if x]
 
     import torch
     lsx = load_package()
     from linearsolve_jl_amd import _abi
+    import linearsolve_jl_amd.problems  # noqa: F401  (lsx.problems)
 
     n, nrhs = args.n, args.nrhs
     log(f"generating A ({n}x{n}) and B ({n}x{nrhs})")
@@ -149,6 +369,9 @@ def main():
     ipiv = np.empty(n, dtype=np.int64)
     ctx.check(L.lsx_lu_get_factors(h, None, n, ipiv.ctypes.data), "lsx_lu_get_factors")
     log(f"backward error {be:.3e}")
+    L.lsx_lu_destroy(h)
+    del dA, dB, dX
+    torch.cuda.empty_cache()
 
     flops = luflop(n) + 2.0 * n * n * nrhs
     ms = t / args.steps * 1e3
@@ -193,7 +416,14 @@ def main():
         result["cpu_baseline"] = cb
         if lapack_ipiv is not None:
             result["pivots_bit_exact_vs_lapack"] = bool(np.array_equal(ipiv, lapack_ipiv))
-    L.lsx_lu_destroy(h)
+    del G, Gb, A, B
+    if "mixed" in sections:
+        result["mixed"] = dict(n=args.mixed_n, nrhs=1, tol=1e-13,
+                               runs=run_mixed(lsx, _abi, ctx, L, args.mixed_n, (1e2, 1e4, 1e6), 2))
+    if "batched" in sections:
+        result["batched"] = run_batched(lsx, _abi, ctx, L, args.batch_count, 3, 300)
+    if "serving" in sections:
+        result["serving"] = run_serving(lsx, args.serve_seconds, args.serve_rate, args.serve_gmres_maxiters, 64)
     print(json.dumps(result), flush=True)
 
 

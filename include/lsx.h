@@ -127,14 +127,26 @@ int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double
 /* count independent systems of sizes n[i] (1 <= n[i] <= 512), packed back to back:
  * A_i column-major at offset sum_{j<i} n_j^2, b_i / x_i at offset sum_{j<i} n_j.
  * Replaces the per-block lu!/ldiv! loop of BlockDiagonalFactorization
- * (ext/LinearSolveBlockDiagonalsExt.jl:101-114,123-185). */
+ * (ext/LinearSolveBlockDiagonalsExt.jl:101-114,123-185).  dtype LSX_F64 = LAPACK dgetrf/dgetrs
+ * per block; LSX_F32 = FP32 factorization with FP64 iterative refinement of each solution
+ * (no reference counterpart: the FP32 analogue of the blockwise path). */
 int lsx_batch_create(lsx_ctx* ctx, int64_t count, const int64_t* n, int dtype, lsx_batch** out);
 int lsx_batch_destroy(lsx_batch* batch);
-/* Factor and solve every system in one persistent launch.  info[i] (nullable, host or device)
- * receives each system's LAPACK info; ipiv (nullable) receives the pivots packed like b.
- * Returns the number of systems with info > 0 (0 = all factored). */
+/* Factor and solve every system in one persistent launch (replayed from a HIP graph).
+ * info[i] (nullable, host or device) receives each system's LAPACK info; ipiv (nullable)
+ * receives the pivots packed like b.  Returns the number of systems with info > 0. */
 int lsx_batch_factor_solve(lsx_batch* batch, const double* A, const double* b, double* x,
                            int64_t* info, int64_t* ipiv);
+/* Size-binned tile table: a system of size n uses panel width bw[k] (8 or 16) of the first bin
+ * with n <= upper[k]; sizes beyond the last bin keep the built-in choice.  Chosen by the autotuner (linearsolve.jl_amd/autotune.py); no reference
+ * counterpart. */
+int lsx_batch_set_tiles(lsx_batch* batch, int32_t nbins, const int32_t* upper, const int32_t* bw);
+/* LSX_F32 batches: refinement target (normwise backward error, default 1e-14) and step limit
+ * (default 10).  No reference counterpart. */
+int lsx_batch_set_refinement(lsx_batch* batch, double tol, int32_t max_iters);
+/* Per-system refinement steps and final backward error of the last factor_solve (LSX_F32;
+ * LSX_F64 batches report 0 steps and -1).  Either pointer may be NULL.  No reference counterpart. */
+int lsx_batch_stats(lsx_batch* batch, int32_t* iters, double* backward_error);
 
 /* ---------------------------------------------------------------- sparse / SimpleGMRES --- */
 /* CSR matrix resident on the device (int32 row pointers and column indices, FP64 values). */

@@ -13,8 +13,10 @@ Reference interface (paths in LinearSolve.jl v5.12.0):
 GPU; this module only moves arrays, keeps handles in `cache.cacheval` and maps return codes.
 """
 import ctypes
+import json
 import enum
 import math
+import os
 
 import numpy as np
 
@@ -366,20 +368,51 @@ class BlockDiagonal:
         return D
 
 
+TILE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
+
+
+def load_tile_table(path=TILE_TABLE):
+    """Size-binned panel widths written by autotune.py: {"bins": [[upper, bw], ...]} (None if absent)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return [(int(u), int(w)) for u, w in d["bins"]]
+
+
 class _BatchHandle:
-    def __init__(self, ctx, sizes, dtype):
+    def __init__(self, ctx, sizes, dtype, tiles="auto"):
         self.ctx = ctx
         self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
         h = ctypes.c_void_p()
         ctx.check(_abi.lib().lsx_batch_create(ctx.handle, len(self.sizes), self.sizes.ctypes.data, dtype,
                                               ctypes.byref(h)), "lsx_batch_create")
         self.h = h
+        table = load_tile_table() if tiles == "auto" else tiles
+        if table:
+            self.set_tiles(table)
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value and _abi._lib is not None:
             _abi.lib().lsx_batch_destroy(h)
             self.h = None
+
+    def set_tiles(self, table):
+        up = np.ascontiguousarray([u for u, _ in table], dtype=np.int32)
+        bw = np.ascontiguousarray([w for _, w in table], dtype=np.int32)
+        self.ctx.check(_abi.lib().lsx_batch_set_tiles(self.h, len(up), up.ctypes.data, bw.ctypes.data),
+                       "lsx_batch_set_tiles")
+
+    def set_refinement(self, tol, max_iters):
+        self.ctx.check(_abi.lib().lsx_batch_set_refinement(self.h, float(tol), int(max_iters)),
+                       "lsx_batch_set_refinement")
+
+    def stats(self):
+        it = np.zeros(len(self.sizes), dtype=np.int32)
+        be = np.zeros(len(self.sizes), dtype=np.float64)
+        self.ctx.check(_abi.lib().lsx_batch_stats(self.h, it.ctypes.data, be.ctypes.data), "lsx_batch_stats")
+        return it, be
 
     def factor_solve(self, Apacked, b, x, want_pivots=False):
         info = np.zeros(len(self.sizes), dtype=np.int64)
@@ -395,23 +428,36 @@ class _BatchHandle:
 class MI355XBatchedLUFactorization(_DenseAlg):
     """Drop-in for the blockwise LU of BlockDiagonalFactorization
     (ext/LinearSolveBlockDiagonalsExt.jl:48-185): every block factored and solved in one
-    persistent launch.  success = all blocks nonsingular (:101-114)."""
+    persistent launch.  success = all blocks nonsingular (:101-114).  precision="fp32" factors in
+    FP32 and refines every block's solution in FP64 to backward error <= tol; success then also
+    needs every block to reach tol."""
 
-    def __init__(self, precision="fp64"):
+    def __init__(self, precision="fp64", tol=1e-14, max_iters=10, tiles="auto"):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
         self.dtype = LSX_F64 if precision == "fp64" else LSX_F32
+        self.tol, self.max_iters, self.tiles = tol, max_iters, tiles
         self.last_info = None
         self.last_ipiv = None
+        self.last_iters = None
+        self.last_backward_error = None
 
     def solve_(self, cache):
         A = cache.A
         if not isinstance(A, BlockDiagonal):
             raise ValueError("MI355XBatchedLUFactorization takes a BlockDiagonal A")
         if cache.isfresh or cache.cacheval is None:
-            cache.cacheval = _BatchHandle(cache.ctx, A.sizes, self.dtype)
+            cache.cacheval = _BatchHandle(cache.ctx, A.sizes, self.dtype, self.tiles)
+            if self.dtype == LSX_F32:
+                cache.cacheval.set_refinement(self.tol, self.max_iters)
             object.__setattr__(cache, "isfresh", False)
         nfail, info, ipiv = cache.cacheval.factor_solve(A.packed(), cache.b, cache.u, want_pivots=True)
         self.last_info, self.last_ipiv = info, ipiv
         rc = ReturnCode.Success if nfail == 0 else ReturnCode.Failure
+        if self.dtype == LSX_F32:
+            self.last_iters, self.last_backward_error = cache.cacheval.stats()
+            if nfail == 0 and not np.all(self.last_backward_error <= self.tol):
+                rc = ReturnCode.Failure
         return LinearSolution(cache.u, None, rc, 0, cache)
 
 

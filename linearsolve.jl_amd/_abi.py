@@ -88,6 +88,9 @@ SIGNATURES = [
     ("lsx_batch_create", ctypes.c_int, [_P, _I64, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     ("lsx_batch_destroy", ctypes.c_int, [_P]),
     ("lsx_batch_factor_solve", ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("lsx_batch_set_tiles", ctypes.c_int, [_P, ctypes.c_int32, _P, _P]),
+    ("lsx_batch_set_refinement", ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int32]),
+    ("lsx_batch_stats", ctypes.c_int, [_P, _P, _P]),
     ("lsx_csr_create", ctypes.c_int, [_P, _I64, _I64, _P, _P, _P, ctypes.POINTER(_P)]),
     ("lsx_csr_destroy", ctypes.c_int, [_P]),
     ("lsx_csr_spmv", ctypes.c_int, [_P, _P, _P]),

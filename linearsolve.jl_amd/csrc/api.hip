@@ -23,6 +23,10 @@ int lu64_alloc(lsx_lu* lu) {
     lu->ld = pad_ld(n);
     const int nblk = (int)cdiv(n, DB);
     LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream, hipStreamNonBlocking));
+    LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
+    LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_next, hipEventDisableTiming));
+    LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_panel, hipEventDisableTiming));
+    lu->cur = lu->stream;
     if (hipMalloc(&lu->F, sizeof(double) * lu->ld * n) != hipSuccess)
         return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate %lld x %lld FP64 factors", (long long)n,
                     (long long)n);
@@ -31,8 +35,12 @@ int lu64_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&lu->dinfo, sizeof(int64_t) * 2));
     LSX_HIP(ctx, hipHostMalloc(&lu->hinfo, sizeof(int64_t) * 2, hipHostMallocDefault));
     LSX_HIP(ctx, hipMalloc(&lu->slots, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS));
-    LSX_HIP(ctx, hipMalloc(&lu->plist, sizeof(int64_t) * 8 * NBO));
-    LSX_HIP(ctx, hipMalloc(&lu->pcount, sizeof(int) * 2));
+    LSX_HIP(ctx, hipMalloc(&lu->plist, sizeof(int64_t) * 12 * NBO));
+    LSX_HIP(ctx, hipMalloc(&lu->pcount, sizeof(int) * 3));
+    // inverse row maps of the deferred L-column interchanges (used when a column segment of the
+    // working type fits in LDS; FP32 is the smaller type)
+    if ((n - NBO) * (int64_t)sizeof(float) <= LGMAX_BYTES && n > NBO)
+        LSX_HIP(ctx, hipMalloc(&lu->pinv, sizeof(int) * cdiv(n, NBO) * n));
     LSX_HIP(ctx, hipMalloc(&lu->tflags, sizeof(uint32_t) * 2 * nblk));
     LSX_HIP(ctx, hipMalloc(&lu->DL, sizeof(double) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&lu->DU, sizeof(double) * DB * DB * nblk));
@@ -53,11 +61,16 @@ void lu64_free(lsx_lu* lu) {
     (void)hipFree(lu->slots);
     (void)hipFree(lu->plist);
     (void)hipFree(lu->pcount);
+    (void)hipFree(lu->pinv);
     (void)hipFree(lu->tflags);
     (void)hipFree(lu->DL);
     (void)hipFree(lu->DU);
     (void)hipFree(lu->wb);
     (void)hipFree(lu->wy);
+    if (lu->stream2) (void)hipStreamSynchronize(lu->stream2);
+    if (lu->ev_next) (void)hipEventDestroy(lu->ev_next);
+    if (lu->ev_panel) (void)hipEventDestroy(lu->ev_panel);
+    if (lu->stream2) (void)hipStreamDestroy(lu->stream2);
     if (lu->stream) (void)hipStreamDestroy(lu->stream);
 }
 }  // namespace lsx

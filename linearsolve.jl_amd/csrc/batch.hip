@@ -3,19 +3,28 @@
 //
 // Reference: the blockwise factorization of BlockDiagonalFactorization
 // (ext/LinearSolveBlockDiagonalsExt.jl:101-114: lu!(B_i, pivot; check = false) per block,
-// success = all(issuccess); :123-185 per-block ldiv!).  Per system the arithmetic is that of
-// LAPACK dgetrf/dgetrs (pivot = first index of max |a|, NaN ignored; info = first zero pivot).
+// success = all(issuccess); :123-185 per-block ldiv!).  Per system the arithmetic of the FP64
+// path is that of LAPACK dgetrf/dgetrs (pivot = first index of max |a|, NaN ignored; info = first
+// zero pivot).  The FP32 path factors in FP32 and refines the solution in FP64
+// (r = b - A x with the FP64 matrix, d = U \ L \ P r, x += d) until the normwise backward error
+// ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) reaches the tolerance.
 //
 // Scheduling: systems are sorted by decreasing size on the host (longest first); a persistent
 // grid of workgroups takes them from an atomic counter, one workgroup per system.
 // Per system (BT threads): right-looking blocked LU with panels of BW columns held in LDS; rows
 // are never moved in memory: a per-system permutation perm[] (logical row -> physical row)
 // absorbs the interchanges, so a swap outside the panel is a swap of two perm entries.  The
-// trailing update A22 -= L21 U12 runs on FP64 MFMA with L21 from LDS and U12 from L2.
-// The factors stay in the workspace in physical row order together with perm (for re-solves);
-// pivots are reported as the LAPACK swap sequence.
+// trailing update A22 -= L21 U12 runs on MFMA (FP64 16x16x4 or FP32 16x16x4) with L21 from LDS
+// and U12 from L2.  The panel width BW (8 or 16) is chosen per system from a size-binned tile
+// table (set by the autotuner through lsx_batch_set_tiles).  Triangular solves go by blocks of 64
+// rows: the diagonal block is staged in LDS and solved by one wave with shuffles, the rows
+// beyond it are updated by all threads.
+// The launch sequence of a handle (counter reset + persistent kernel) is captured once in a HIP
+// graph and replayed by every factor_solve call (LSX_BATCH_NO_GRAPH=1 launches it directly).
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
+#include <limits>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -25,32 +34,45 @@
 namespace lsx {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int BT = 256;       // threads per system
-constexpr int BW = 16;        // panel width
-constexpr int BNMAX = 512;    // largest system
+constexpr int BT = 256;            // threads per system
+constexpr int BNMAX = 512;         // largest system
+constexpr int PBUF = BNMAX * 17;   // panel buffer in doubles: 512 x (16 + 1) or 512 x (8 + 1)
+constexpr int SB = 64;             // row block of the triangular solves (one wave)
+
+// Panel-width index of a system without a tile table.
+__host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : 1; }
 
 struct BatchArgs {
     int64_t count;
     const int32_t* order;     // systems by decreasing n
     const int32_t* n;
-    const int64_t* aoff;      // offset of A_i (n_i^2 doubles)
+    const uint8_t* cfg;       // per-system panel-width index (nullable: default by n)
+    const int64_t* aoff;      // offset of A_i (n_i^2 elements)
     const int64_t* voff;      // offset of b_i / x_i / ipiv_i
-    double* A;                // workspace: factored in place
+    void* A;                  // workspace of the factor type: factored in place
+    const double* A64;        // FP32 path: the FP64 matrices (refinement residuals)
     const double* b;
     double* x;
     int64_t* ipiv;            // LAPACK swap sequence (1-based), packed like b
     int32_t* perm;            // logical -> physical rows, packed like b
     int64_t* info;
+    int32_t* iters;           // refinement steps per system (nullable)
+    double* berr;             // FP32 path: final backward error per system (nullable)
+    int maxref;
+    double tol;
     uint32_t* counter;
 };
 
 struct BatchShared {
-    double P[BNMAX][BW + 1];   // panel rows (logical order j0..n)
+    double pbuf[PBUF];         // panel (factor type, row stride BW + 1) / staged diagonal blocks
     double wkey[BT / 64];
     int wrow[BT / 64];
     int prow;
+    double red[2][BT / 64];
     double xs[BNMAX];          // solve vector
+    double xr[BNMAX];          // FP32 path: the refined solution
     int perm[BNMAX];
 };
 
@@ -58,10 +80,222 @@ __device__ __forceinline__ bool bbetter(double ka, int ra, double kb, int rb) {
     return ka > kb || (ka == kb && ra < rb);
 }
 
-__global__ __launch_bounds__(BT) void k_batch_lu(BatchArgs a) {
+// MFMA of the factor type: the transposed 16 x 16 tile D = X Y (X(i, k) = -U12(k, col i),
+// Y(k, j) = L21(row j, k)); lane l / slot q holds C(row rb + (l & 15), col cb + ccol(l, q)).
+template <typename T> struct BMma;
+template <> struct BMma<double> {
+    typedef v4d acc_t;
+    __device__ static acc_t mma(double x, double y, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0); }
+    __device__ static int ccol(int lane, int q) { return 4 * q + (lane >> 4); }
+};
+template <> struct BMma<float> {
+    typedef v4f acc_t;
+    __device__ static acc_t mma(float x, float y, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0); }
+    __device__ static int ccol(int lane, int q) { return 4 * (lane >> 4) + q; }
+};
+
+// Block-wide max of two non-negative values (all threads get both).
+__device__ __forceinline__ void block_max2(BatchShared& sh, double& a, double& b) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off));
+        b = fmax(b, __shfl_xor(b, off));
+    }
+    if (lane == 0) { sh.red[0][wave] = a; sh.red[1][wave] = b; }
+    __syncthreads();
+    a = sh.red[0][0];
+    b = sh.red[1][0];
+#pragma unroll
+    for (int i = 1; i < BT / 64; ++i) { a = fmax(a, sh.red[0][i]); b = fmax(b, sh.red[1][i]); }
+    __syncthreads();
+}
+
+// LU with partial pivoting of one n x n system (column-major, in place, rows through sh.perm).
+template <typename T, int BW>
+__device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T (*P)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(sh.pbuf);
+    for (int j0 = 0; j0 < n; j0 += BW) {
+        const int jb = min(BW, n - j0);
+        const int m = n - j0;
+        // load the panel (logical rows j0..n-1 through perm)
+        for (int idx = tid; idx < m * BW; idx += BT) {
+            const int r = idx % m, c = idx / m;
+            P[r][c] = (c < jb) ? A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] : T(0);
+        }
+        __syncthreads();
+        for (int c = 0; c < jb; ++c) {
+            // pivot: first row of max |a| among logical rows j0+c..n-1 (NaN never wins)
+            double key = -1.0;
+            int krow = 0x7fffffff;
+            for (int r = c + tid; r < m; r += BT) {
+                const double t = fabs((double)P[r][c]);
+                const double kk = (t == t) ? t : 0.0;
+                if (bbetter(kk, r, key, krow)) { key = kk; krow = r; }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ok = __shfl_xor(key, off);
+                const int orow = __shfl_xor(krow, off);
+                if (bbetter(ok, orow, key, krow)) { key = ok; krow = orow; }
+            }
+            if (lane == 0) { sh.wkey[wave] = key; sh.wrow[wave] = krow; }
+            __syncthreads();
+            if (tid == 0) {
+                double bk = sh.wkey[0];
+                int br = sh.wrow[0];
+                for (int i = 1; i < BT / 64; ++i)
+                    if (bbetter(sh.wkey[i], sh.wrow[i], bk, br)) { bk = sh.wkey[i]; br = sh.wrow[i]; }
+                sh.prow = br;
+                const int k = j0 + c;
+                if (ipiv) ipiv[vo + k] = (int64_t)(j0 + br) + 1;
+                if (br != c) {
+                    const int t = sh.perm[k];
+                    sh.perm[k] = sh.perm[j0 + br];
+                    sh.perm[j0 + br] = t;
+                }
+            }
+            __syncthreads();
+            const int pr = sh.prow;
+            if (pr != c && tid < BW) {
+                const T t = P[c][tid];
+                P[c][tid] = P[pr][tid];
+                P[pr][tid] = t;
+            }
+            __syncthreads();
+            const T pv = P[c][c];
+            if (pv == T(0) && info == 0) info = j0 + c + 1;
+            for (int r = c + 1 + tid; r < m; r += BT) {
+                T l = P[r][c];
+                if (pv != T(0)) {
+                    if (fabs((double)pv) >= (double)std::numeric_limits<T>::min()) l *= (T(1) / pv);
+                    else l /= pv;
+                }
+                P[r][c] = l;
+                for (int q = c + 1; q < jb; ++q) P[r][q] = fma(-l, P[c][q], P[r][q]);
+            }
+            __syncthreads();
+        }
+        // write the panel back (physical rows through the updated perm)
+        for (int idx = tid; idx < m * jb; idx += BT) {
+            const int r = idx % m, c = idx / m;
+            A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] = P[r][c];
+        }
+        const int c1 = j0 + jb;
+        if (c1 < n) {
+            // U12 = L11^{-1} A12, one thread per column
+            for (int col = c1 + tid; col < n; col += BT) {
+                T u[BW];
+#pragma unroll
+                for (int i = 0; i < BW; ++i) u[i] = (i < jb) ? A[(int64_t)col * n + sh.perm[j0 + i]] : T(0);
+#pragma unroll
+                for (int kk = 0; kk < BW; ++kk) {
+#pragma unroll
+                    for (int i = kk + 1; i < BW; ++i) u[i] = fma(-P[i][kk], u[kk], u[i]);
+                    __builtin_amdgcn_sched_barrier(0);  // L11 reads column by column, not all hoisted
+                }
+#pragma unroll
+                for (int i = 0; i < BW; ++i)
+                    if (i < jb) A[(int64_t)col * n + sh.perm[j0 + i]] = u[i];
+            }
+            __syncthreads();
+            // A22 -= L21 U12 on MFMA: 16 x 16 tiles, waves round-robin
+            const int mr = n - c1, nc = n - c1;
+            const int tr = (mr + 15) / 16, tc = (nc + 15) / 16;
+            for (int t = wave; t < tr * tc; t += BT / 64) {
+                const int rb = (t % tr) * 16, cb = (t / tr) * 16;
+                const int lr = rb + (lane & 15);  // local row of C
+                const bool rok = lr < mr;
+                const int prow = rok ? sh.perm[c1 + lr] : 0;
+                typename BMma<T>::acc_t acc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int lc = cb + BMma<T>::ccol(lane, q);
+                    acc[q] = (rok && lc < nc) ? A[(int64_t)(c1 + lc) * n + prow] : T(0);
+                }
+#pragma unroll
+                for (int k4 = 0; k4 < BW / 4; ++k4) {
+                    const int kk = k4 * 4 + (lane >> 4);
+                    const int lc = cb + (lane & 15);
+                    const T xv = (lc < nc && kk < jb) ? -A[(int64_t)(c1 + lc) * n + sh.perm[j0 + kk]] : T(0);
+                    const T yv = (rok && kk < jb) ? P[jb + lr][kk] : T(0);
+                    acc = BMma<T>::mma(xv, yv, acc);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int lc = cb + BMma<T>::ccol(lane, q);
+                    if (rok && lc < nc) A[(int64_t)(c1 + lc) * n + prow] = acc[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// sh.xs := U^{-1} L^{-1} sh.xs (sh.xs already holds P b), arithmetic in FP64.
+template <typename T>
+__device__ void solve_sys(BatchShared& sh, const T* A, int n) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double (*D)[SB + 1] = reinterpret_cast<double (*)[SB + 1]>(sh.pbuf);
+    // forward: unit lower L
+    for (int k0 = 0; k0 < n; k0 += SB) {
+        const int kb = min(SB, n - k0);
+        for (int idx = tid; idx < kb * kb; idx += BT) {
+            const int r = idx % kb, c = idx / kb;
+            D[r][c] = (double)A[(int64_t)(k0 + c) * n + sh.perm[k0 + r]];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double xv = lane < kb ? sh.xs[k0 + lane] : 0.0;
+            for (int c = 0; c < kb; ++c) {
+                const double xc = __shfl(xv, c);
+                if (lane > c && lane < kb) xv = fma(-D[lane][c], xc, xv);
+            }
+            if (lane < kb) sh.xs[k0 + lane] = xv;
+        }
+        __syncthreads();
+        for (int i = k0 + kb + tid; i < n; i += BT) {
+            double s = sh.xs[i];
+            const int pi = sh.perm[i];
+            for (int k = k0; k < k0 + kb; ++k) s = fma(-(double)A[(int64_t)k * n + pi], sh.xs[k], s);
+            sh.xs[i] = s;
+        }
+        __syncthreads();
+    }
+    // backward: U
+    for (int k1 = n; k1 > 0; k1 -= SB) {
+        const int k0 = max(0, k1 - SB), kb = k1 - k0;
+        for (int idx = tid; idx < kb * kb; idx += BT) {
+            const int r = idx % kb, c = idx / kb;
+            D[r][c] = (double)A[(int64_t)(k0 + c) * n + sh.perm[k0 + r]];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double xv = lane < kb ? sh.xs[k0 + lane] : 0.0;
+            for (int c = kb - 1; c >= 0; --c) {
+                if (lane == c) xv = xv / D[c][c];
+                const double xc = __shfl(xv, c);
+                if (lane < c) xv = fma(-D[lane][c], xc, xv);
+            }
+            if (lane < kb) sh.xs[k0 + lane] = xv;
+        }
+        __syncthreads();
+        for (int i = tid; i < k0; i += BT) {
+            double s = sh.xs[i];
+            const int pi = sh.perm[i];
+            for (int k = k0; k < k1; ++k) s = fma(-(double)A[(int64_t)k * n + pi], sh.xs[k], s);
+            sh.xs[i] = s;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
     __shared__ BatchShared sh;
     __shared__ int s_item;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     while (true) {
         if (tid == 0) s_item = (int)atomicAdd(a.counter, 1u);
         __syncthreads();
@@ -69,168 +303,70 @@ __global__ __launch_bounds__(BT) void k_batch_lu(BatchArgs a) {
         if (item >= a.count) break;
         const int sys = a.order[item];
         const int n = a.n[sys];
-        double* A = a.A + a.aoff[sys];
+        T* A = reinterpret_cast<T*>(a.A) + a.aoff[sys];
         const int64_t vo = a.voff[sys];
         int64_t info = 0;
+        const double* A64 = a.A64 ? a.A64 + a.aoff[sys] : nullptr;
+        if (A64) {  // FP32 path: working copy of the FP64 matrix
+            for (int64_t i = tid; i < (int64_t)n * n; i += BT) A[i] = (T)A64[i];
+        }
         for (int i = tid; i < n; i += BT) sh.perm[i] = i;
         __syncthreads();
-        for (int j0 = 0; j0 < n; j0 += BW) {
-            const int jb = min(BW, n - j0);
-            const int m = n - j0;
-            // load the panel (logical rows j0..n-1 through perm)
-            for (int idx = tid; idx < m * BW; idx += BT) {
-                const int r = idx % m, c = idx / m;
-                sh.P[r][c] = (c < jb) ? A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] : 0.0;
-            }
-            __syncthreads();
-            for (int c = 0; c < jb; ++c) {
-                // pivot: first row of max |a| among logical rows j0+c..n-1 (NaN never wins)
-                double key = -1.0;
-                int krow = 0x7fffffff;
-                for (int r = c + tid; r < m; r += BT) {
-                    const double t = fabs(sh.P[r][c]);
-                    const double kk = (t == t) ? t : 0.0;
-                    if (bbetter(kk, r, key, krow)) { key = kk; krow = r; }
-                }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ok = __shfl_xor(key, off);
-                    const int orow = __shfl_xor(krow, off);
-                    if (bbetter(ok, orow, key, krow)) { key = ok; krow = orow; }
-                }
-                if (lane == 0) { sh.wkey[wave] = key; sh.wrow[wave] = krow; }
-                __syncthreads();
-                if (tid == 0) {
-                    double bk = sh.wkey[0];
-                    int br = sh.wrow[0];
-                    for (int i = 1; i < BT / 64; ++i)
-                        if (bbetter(sh.wkey[i], sh.wrow[i], bk, br)) { bk = sh.wkey[i]; br = sh.wrow[i]; }
-                    sh.prow = br;
-                    const int k = j0 + c;
-                    if (a.ipiv) a.ipiv[vo + k] = (int64_t)(j0 + br) + 1;
-                    if (br != c) {
-                        const int t = sh.perm[k];
-                        sh.perm[k] = sh.perm[j0 + br];
-                        sh.perm[j0 + br] = t;
-                    }
-                }
-                __syncthreads();
-                const int pr = sh.prow;
-                if (pr != c && tid < BW) {
-                    const double t = sh.P[c][tid];
-                    sh.P[c][tid] = sh.P[pr][tid];
-                    sh.P[pr][tid] = t;
-                }
-                __syncthreads();
-                const double pv = sh.P[c][c];
-                if (pv == 0.0 && info == 0) info = j0 + c + 1;
-                for (int r = c + 1 + tid; r < m; r += BT) {
-                    double l = sh.P[r][c];
-                    if (pv != 0.0) {
-                        if (fabs(pv) >= DBL_MIN) l *= (1.0 / pv);
-                        else l /= pv;
-                    }
-                    sh.P[r][c] = l;
-                    for (int q = c + 1; q < jb; ++q) sh.P[r][q] = fma(-l, sh.P[c][q], sh.P[r][q]);
-                }
-                __syncthreads();
-            }
-            // write the panel back (physical rows through the updated perm)
-            for (int idx = tid; idx < m * jb; idx += BT) {
-                const int r = idx % m, c = idx / m;
-                A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] = sh.P[r][c];
-            }
-            const int c1 = j0 + jb;
-            if (c1 < n) {
-                // U12 = L11^{-1} A12, one thread per column
-                for (int col = c1 + tid; col < n; col += BT) {
-                    double u[BW];
-#pragma unroll
-                    for (int i = 0; i < BW; ++i) u[i] = (i < jb) ? A[(int64_t)col * n + sh.perm[j0 + i]] : 0.0;
-#pragma unroll
-                    for (int kk = 0; kk < BW; ++kk)
-#pragma unroll
-                        for (int i = kk + 1; i < BW; ++i) u[i] = fma(-sh.P[i][kk], u[kk], u[i]);
-#pragma unroll
-                    for (int i = 0; i < BW; ++i)
-                        if (i < jb) A[(int64_t)col * n + sh.perm[j0 + i]] = u[i];
-                }
-                __syncthreads();
-                // A22 -= L21 U12 on MFMA: 16 x 16 tiles, waves round-robin.  The MFMA computes
-                // the transposed tile, so lane l / slot q holds C(rb + (l & 15), cb + 4q + (l >> 4)).
-                const int mr = n - c1, nc = n - c1;
-                const int tr = (mr + 15) / 16, tc = (nc + 15) / 16;
-                for (int t = wave; t < tr * tc; t += BT / 64) {
-                    const int rb = (t % tr) * 16, cb = (t / tr) * 16;
-                    const int lr = rb + (lane & 15);                 // local row of C
-                    const bool rok = lr < mr;
-                    const int prow = rok ? sh.perm[c1 + lr] : 0;
-                    v4d acc;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int lc = cb + 4 * q + (lane >> 4);
-                        acc[q] = (rok && lc < nc) ? A[(int64_t)(c1 + lc) * n + prow] : 0.0;
-                    }
-#pragma unroll
-                    for (int k4 = 0; k4 < BW / 4; ++k4) {
-                        const int kk = k4 * 4 + (lane >> 4);
-                        const int lc = cb + (lane & 15);
-                        // X(i, k) = -U12(k, col i), Y(k, j) = L21(row j, k)
-                        const double xv = (lc < nc && kk < jb) ? -A[(int64_t)(c1 + lc) * n + sh.perm[j0 + kk]] : 0.0;
-                        const double yv = (rok && kk < jb) ? sh.P[jb + lr][kk] : 0.0;
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, yv, acc, 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int lc = cb + 4 * q + (lane >> 4);
-                        if (rok && lc < nc) A[(int64_t)(c1 + lc) * n + prow] = acc[q];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // solve: x = U^{-1} L^{-1} P b, blocked by BW rows
+        const int cfg = a.cfg ? a.cfg[sys] : default_cfg(n);
+        if (cfg == 0) factor_sys<T, 8>(sh, A, n, vo, a.ipiv, info);
+        else factor_sys<T, 16>(sh, A, n, vo, a.ipiv, info);
+        // solve: x = U^{-1} L^{-1} P b
         for (int i = tid; i < n; i += BT) sh.xs[i] = a.b[vo + sh.perm[i]];
         __syncthreads();
-        for (int k0 = 0; k0 < n; k0 += BW) {
-            const int kb = min(BW, n - k0);
-            if (tid == 0) {
-                for (int k = k0; k < k0 + kb; ++k) {
-                    const double xk = sh.xs[k];
-                    for (int i = k + 1; i < k0 + kb; ++i) sh.xs[i] = fma(-A[(int64_t)k * n + sh.perm[i]], xk, sh.xs[i]);
+        solve_sys<T>(sh, A, n);
+        if (A64) {
+            // FP64 refinement
+            double an = 0.0, bn = 0.0;
+            for (int i = tid; i < n; i += BT) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j) s += fabs(A64[(int64_t)j * n + i]);
+                an = fmax(an, s);
+                bn = fmax(bn, fabs(a.b[vo + i]));
+            }
+            block_max2(sh, an, bn);
+            for (int i = tid; i < n; i += BT) sh.xr[i] = sh.xs[i];
+            __syncthreads();
+            double* rr = sh.pbuf;  // residual by original row
+            int it = 0;
+            double be = 0.0;
+            while (true) {
+                double rn = 0.0, xn = 0.0;
+                for (int i = tid; i < n; i += BT) {
+                    double s = a.b[vo + i];
+                    for (int j = 0; j < n; ++j) s = fma(-A64[(int64_t)j * n + i], sh.xr[j], s);
+                    rr[i] = s;
+                    rn = fmax(rn, fabs(s));
+                    xn = fmax(xn, fabs(sh.xr[i]));
                 }
+                block_max2(sh, rn, xn);
+                const double den = an * xn + bn;
+                be = den > 0.0 ? rn / den : 0.0;
+                if (!(be > a.tol) || it >= a.maxref || info != 0) break;
+                for (int i = tid; i < n; i += BT) sh.xs[i] = rr[sh.perm[i]];
+                __syncthreads();
+                solve_sys<T>(sh, A, n);
+                for (int i = tid; i < n; i += BT) sh.xr[i] += sh.xs[i];
+                __syncthreads();
+                ++it;
             }
-            __syncthreads();
-            for (int i = k0 + kb + tid; i < n; i += BT) {
-                double s = sh.xs[i];
-                const int pi = sh.perm[i];
-                for (int k = k0; k < k0 + kb; ++k) s = fma(-A[(int64_t)k * n + pi], sh.xs[k], s);
-                sh.xs[i] = s;
-            }
-            __syncthreads();
-        }
-        for (int k1 = n; k1 > 0; k1 -= BW) {
-            const int k0 = max(0, k1 - BW);
+            for (int i = tid; i < n; i += BT) a.x[vo + i] = sh.xr[i];
             if (tid == 0) {
-                for (int k = k1 - 1; k >= k0; --k) {
-                    const double xk = sh.xs[k] / A[(int64_t)k * n + sh.perm[k]];
-                    sh.xs[k] = xk;
-                    for (int i = k0; i < k; ++i) sh.xs[i] = fma(-A[(int64_t)k * n + sh.perm[i]], xk, sh.xs[i]);
-                }
+                if (a.iters) a.iters[sys] = it;
+                if (a.berr) a.berr[sys] = be;
             }
-            __syncthreads();
-            for (int i = tid; i < k0; i += BT) {
-                double s = sh.xs[i];
-                const int pi = sh.perm[i];
-                for (int k = k0; k < k1; ++k) s = fma(-A[(int64_t)k * n + pi], sh.xs[k], s);
-                sh.xs[i] = s;
+        } else {
+            for (int i = tid; i < n; i += BT) a.x[vo + i] = sh.xs[i];
+            if (tid == 0) {
+                if (a.iters) a.iters[sys] = 0;
+                if (a.berr) a.berr[sys] = -1.0;
             }
-            __syncthreads();
         }
-        for (int i = tid; i < n; i += BT) {
-            a.x[vo + i] = sh.xs[i];
-            a.perm[vo + i] = sh.perm[i];
-        }
+        for (int i = tid; i < n; i += BT) a.perm[vo + i] = sh.perm[i];
         if (tid == 0) a.info[sys] = info;
         __syncthreads();
     }
@@ -242,10 +378,11 @@ int batch_launch_raw(lsx_ctx* ctx, hipStream_t st, int64_t count, const int32_t*
                      const int64_t* aoff, const int64_t* voff, double* A, const double* b, double* x, int64_t* ipiv,
                      int32_t* perm, int64_t* info, uint32_t* counter) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
-    BatchArgs a;
+    BatchArgs a{};
     a.count = count;
     a.order = order;
     a.n = n;
+    a.cfg = nullptr;
     a.aoff = aoff;
     a.voff = voff;
     a.A = A;
@@ -256,34 +393,87 @@ int batch_launch_raw(lsx_ctx* ctx, hipStream_t st, int64_t count, const int32_t*
     a.info = info;
     a.counter = counter;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)ctx->num_cus * 2));
-    hipLaunchKernelGGL(k_batch_lu, dim3(grid), dim3(BT), 0, st, a);
+    hipLaunchKernelGGL(k_batch_lu<double>, dim3(grid), dim3(BT), 0, st, a);
     LSX_HIP(ctx, hipGetLastError());
     return LSX_OK;
+}
+
+// The launch sequence of a handle: counter reset + the persistent kernel on the handle's buffers.
+static void batch_enqueue(lsx_batch* B) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, B->stream, B->counter);
+    BatchArgs a{};
+    a.count = B->count;
+    a.order = B->order;
+    a.n = B->n;
+    a.cfg = B->cfg;
+    a.aoff = B->aoff;
+    a.voff = B->voff;
+    a.b = B->b;
+    a.x = B->x;
+    a.ipiv = B->ipiv;
+    a.perm = B->perm;
+    a.info = B->info;
+    a.iters = B->iters;
+    a.berr = B->berr;
+    a.maxref = B->maxref;
+    a.tol = B->tol;
+    a.counter = B->counter;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(B->count, (int64_t)B->ctx->num_cus * 2));
+    if (B->dtype == LSX_F64) {
+        a.A = B->A;
+        a.A64 = nullptr;
+        hipLaunchKernelGGL(k_batch_lu<double>, dim3(grid), dim3(BT), 0, B->stream, a);
+    } else {
+        a.A = B->A32;
+        a.A64 = B->A;
+        hipLaunchKernelGGL(k_batch_lu<float>, dim3(grid), dim3(BT), 0, B->stream, a);
+    }
 }
 
 int batch_run(lsx_batch* B, const double* dA_src, const double* db, double* dx, bool copy_a) {
     lsx_ctx* ctx = B->ctx;
     if (copy_a)
         LSX_HIP(ctx, hipMemcpyAsync(B->A, dA_src, sizeof(double) * B->total_a, hipMemcpyDefault, B->stream));
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, B->stream, B->counter);
-    BatchArgs a;
-    a.count = B->count;
-    a.order = B->order;
-    a.n = B->n;
-    a.aoff = B->aoff;
-    a.voff = B->voff;
-    a.A = B->A;
-    a.b = db;
-    a.x = dx;
-    a.ipiv = B->ipiv;
-    a.perm = B->perm;
-    a.info = B->info;
-    a.counter = B->counter;
-    const int grid = std::max<int64_t>(1, std::min<int64_t>(B->count, (int64_t)ctx->num_cus * 2));
+    if (db != B->b) LSX_HIP(ctx, hipMemcpyAsync(B->b, db, sizeof(double) * B->total_v, hipMemcpyDefault, B->stream));
     const int pi = B->prof.begin(B->stream, PK_OTHER, B->flops, sizeof(double) * 2.0 * (double)B->total_a);
-    hipLaunchKernelGGL(k_batch_lu, dim3(grid), dim3(BT), 0, B->stream, a);
+    if (B->use_graph) {
+        if (!B->graph_exec) {
+            hipGraph_t g = nullptr;
+            LSX_HIP(ctx, hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            batch_enqueue(B);
+            LSX_HIP(ctx, hipStreamEndCapture(B->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            LSX_HIP(ctx, e);
+        }
+        LSX_HIP(ctx, hipGraphLaunch(B->graph_exec, B->stream));
+    } else {
+        batch_enqueue(B);
+    }
     B->prof.end(B->stream, pi);
     LSX_HIP(ctx, hipGetLastError());
+    if (dx != B->x) LSX_HIP(ctx, hipMemcpyAsync(dx, B->x, sizeof(double) * B->total_v, hipMemcpyDefault, B->stream));
+    return LSX_OK;
+}
+
+// Per-system panel-width indices from a size-binned table (first bin with n <= upper).
+static int tiles_to_cfg(lsx_ctx* ctx, const std::vector<int32_t>& hn, int nbins, const int32_t* upper,
+                        const int32_t* bw, std::vector<uint8_t>& cfg) {
+    for (int k = 0; k < nbins; ++k)
+        if (bw[k] != 8 && bw[k] != 16)
+            return fail(ctx, LSX_EINVAL, "lsx_batch_set_tiles: panel width %d is not 8 or 16", bw[k]);
+    cfg.resize(hn.size());
+    for (size_t i = 0; i < hn.size(); ++i) {
+        const int n = hn[i];
+        int c = default_cfg(n);
+        for (int k = 0; k < nbins; ++k) {
+            if (n <= upper[k]) {
+                c = bw[k] == 8 ? 0 : 1;
+                break;
+            }
+        }
+        cfg[i] = (uint8_t)c;
+    }
     return LSX_OK;
 }
 
@@ -293,16 +483,21 @@ using namespace lsx;
 
 static void batch_free(lsx_batch* B) {
     if (B->stream) (void)hipStreamSynchronize(B->stream);
+    if (B->graph_exec) (void)hipGraphExecDestroy(B->graph_exec);
     (void)hipFree(B->order);
     (void)hipFree(B->n);
+    (void)hipFree(B->cfg);
     (void)hipFree(B->aoff);
     (void)hipFree(B->voff);
     (void)hipFree(B->A);
+    (void)hipFree(B->A32);
     (void)hipFree(B->b);
     (void)hipFree(B->x);
     (void)hipFree(B->ipiv);
     (void)hipFree(B->perm);
     (void)hipFree(B->info);
+    (void)hipFree(B->iters);
+    (void)hipFree(B->berr);
     (void)hipFree(B->counter);
     if (B->hinfo) (void)hipHostFree(B->hinfo);
     if (B->stream) (void)hipStreamDestroy(B->stream);
@@ -313,7 +508,8 @@ extern "C" {
 int lsx_batch_create(lsx_ctx* ctx, int64_t count, const int64_t* n, int dtype, lsx_batch** out) {
     if (!ctx || !out || !n || count < 1) return LSX_EINVAL;
     *out = nullptr;
-    if (dtype != LSX_F64) return fail(ctx, LSX_EINVAL, "lsx_batch_create: only LSX_F64 systems are batched");
+    if (dtype != LSX_F64 && dtype != LSX_F32)
+        return fail(ctx, LSX_EINVAL, "lsx_batch_create: dtype must be LSX_F64 or LSX_F32");
     std::vector<int32_t> hn(count);
     std::vector<int64_t> aoff(count), voff(count);
     int64_t ta = 0, tv = 0;
@@ -333,14 +529,20 @@ int lsx_batch_create(lsx_ctx* ctx, int64_t count, const int64_t* n, int dtype, l
     std::vector<int32_t> order(count);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return hn[x] > hn[y]; });
+    std::vector<uint8_t> cfg(count);
+    for (int64_t i = 0; i < count; ++i) cfg[i] = (uint8_t)default_cfg(hn[i]);
     DeviceGuard g(ctx);
     lsx_batch* B = new (std::nothrow) lsx_batch();
     if (!B) return LSX_ENOMEM;
     B->ctx = ctx;
+    B->dtype = dtype;
     B->count = count;
     B->total_a = ta;
     B->total_v = tv;
     B->flops = flops;
+    B->hn = hn;
+    const char* ng = std::getenv("LSX_BATCH_NO_GRAPH");
+    B->use_graph = !(ng && ng[0] == '1');
     auto bad = [&](int rc) {
         batch_free(B);
         delete B;
@@ -349,17 +551,23 @@ int lsx_batch_create(lsx_ctx* ctx, int64_t count, const int64_t* n, int dtype, l
     if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&B->order, sizeof(int32_t) * count) != hipSuccess ||
         hipMalloc(&B->n, sizeof(int32_t) * count) != hipSuccess ||
+        hipMalloc(&B->cfg, sizeof(uint8_t) * count) != hipSuccess ||
         hipMalloc(&B->aoff, sizeof(int64_t) * count) != hipSuccess ||
         hipMalloc(&B->voff, sizeof(int64_t) * count) != hipSuccess ||
-        hipMalloc(&B->A, sizeof(double) * ta) != hipSuccess || hipMalloc(&B->b, sizeof(double) * tv) != hipSuccess ||
+        hipMalloc(&B->A, sizeof(double) * ta) != hipSuccess ||
+        (dtype == LSX_F32 && hipMalloc(&B->A32, sizeof(float) * ta) != hipSuccess) ||
+        hipMalloc(&B->b, sizeof(double) * tv) != hipSuccess ||
         hipMalloc(&B->x, sizeof(double) * tv) != hipSuccess || hipMalloc(&B->ipiv, sizeof(int64_t) * tv) != hipSuccess ||
         hipMalloc(&B->perm, sizeof(int32_t) * tv) != hipSuccess ||
         hipMalloc(&B->info, sizeof(int64_t) * count) != hipSuccess ||
+        hipMalloc(&B->iters, sizeof(int32_t) * count) != hipSuccess ||
+        hipMalloc(&B->berr, sizeof(double) * count) != hipSuccess ||
         hipMalloc(&B->counter, 256) != hipSuccess ||
         hipHostMalloc(&B->hinfo, sizeof(int64_t) * count, hipHostMallocDefault) != hipSuccess)
         return bad(fail(ctx, LSX_ENOMEM, "lsx_batch_create: device allocation failed"));
     if (hipMemcpy(B->order, order.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->n, hn.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->cfg, cfg.data(), sizeof(uint8_t) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->aoff, aoff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->voff, voff.data(), sizeof(int64_t) * count, hipMemcpyHostToDevice) != hipSuccess)
         return bad(fail(ctx, LSX_EHIP, "lsx_batch_create: upload failed"));
@@ -375,15 +583,36 @@ int lsx_batch_destroy(lsx_batch* B) {
     return LSX_OK;
 }
 
+int lsx_batch_set_tiles(lsx_batch* B, int32_t nbins, const int32_t* upper, const int32_t* bw) {
+    if (!B || nbins < 0 || (nbins > 0 && (!upper || !bw))) return LSX_EINVAL;
+    lsx_ctx* ctx = B->ctx;
+    DeviceGuard g(ctx);
+    std::vector<uint8_t> cfg;
+    const int rc = tiles_to_cfg(ctx, B->hn, nbins, upper, bw, cfg);
+    if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpyAsync(B->cfg, cfg.data(), sizeof(uint8_t) * B->count, hipMemcpyHostToDevice, B->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(B->stream));
+    return LSX_OK;
+}
+
+int lsx_batch_set_refinement(lsx_batch* B, double tol, int32_t max_iters) {
+    if (!B || !(tol >= 0.0) || max_iters < 0) return LSX_EINVAL;
+    B->tol = tol;
+    B->maxref = max_iters;
+    if (B->graph_exec) {  // the values are kernel arguments of the captured launch
+        (void)hipGraphExecDestroy(B->graph_exec);
+        B->graph_exec = nullptr;
+    }
+    return LSX_OK;
+}
+
 int lsx_batch_factor_solve(lsx_batch* B, const double* A, const double* b, double* x, int64_t* info,
                            int64_t* ipiv) {
     if (!B || !A || !b || !x) return LSX_EINVAL;
     lsx_ctx* ctx = B->ctx;
     DeviceGuard g(ctx);
-    LSX_HIP(ctx, hipMemcpyAsync(B->b, b, sizeof(double) * B->total_v, hipMemcpyDefault, B->stream));
-    int rc = batch_run(B, A, B->b, B->x, true);
+    int rc = batch_run(B, A, b, x, true);
     if (rc) return rc;
-    LSX_HIP(ctx, hipMemcpyAsync(x, B->x, sizeof(double) * B->total_v, hipMemcpyDefault, B->stream));
     LSX_HIP(ctx, hipMemcpyAsync(B->hinfo, B->info, sizeof(int64_t) * B->count, hipMemcpyDeviceToHost, B->stream));
     if (info) LSX_HIP(ctx, hipMemcpyAsync(info, B->info, sizeof(int64_t) * B->count, hipMemcpyDefault, B->stream));
     if (ipiv) LSX_HIP(ctx, hipMemcpyAsync(ipiv, B->ipiv, sizeof(int64_t) * B->total_v, hipMemcpyDefault, B->stream));
@@ -392,6 +621,17 @@ int lsx_batch_factor_solve(lsx_batch* B, const double* A, const double* b, doubl
     int nfail = 0;
     for (int64_t i = 0; i < B->count; ++i) nfail += B->hinfo[i] > 0;
     return nfail;
+}
+
+int lsx_batch_stats(lsx_batch* B, int32_t* iters, double* backward_error) {
+    if (!B) return LSX_EINVAL;
+    lsx_ctx* ctx = B->ctx;
+    DeviceGuard g(ctx);
+    if (iters) LSX_HIP(ctx, hipMemcpyAsync(iters, B->iters, sizeof(int32_t) * B->count, hipMemcpyDefault, B->stream));
+    if (backward_error)
+        LSX_HIP(ctx, hipMemcpyAsync(backward_error, B->berr, sizeof(double) * B->count, hipMemcpyDefault, B->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(B->stream));
+    return LSX_OK;
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@ constexpr int REC = 4 + 2 * PW;   // words of one FP64 exchange record: key lo/h
 constexpr int RECS = 72;          // record stride in the exchange buffer (words)
 constexpr int DB = 64;            // diagonal block of the explicit inverses and triangular solves
 constexpr int TRHS = 16;          // right-hand sides per triangular-solve launch
+constexpr int64_t LGMAX_BYTES = 144 * 1024;  // LDS of the deferred L-interchange pass
 }  // namespace lsx
 
 struct lsx_lu {
@@ -17,7 +18,11 @@ struct lsx_lu {
     int64_t n = 0;
     int dtype = LSX_F64;
     int64_t ld = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // trailing updates and everything outside getrf
+    hipStream_t stream2 = nullptr;  // getrf lookahead: panel factorization of the next block
+    hipStream_t cur = nullptr;      // stream the launch helpers use at the moment
+    hipEvent_t ev_next = nullptr;   // next block's columns updated (stream -> stream2)
+    hipEvent_t ev_panel = nullptr;  // next block's panel factored (stream2 -> stream)
     // FP64 factors (also the FP64 copy of A for the mixed path's residuals)
     double* F = nullptr;
     int64_t* ipiv = nullptr;     // device, 1-based
@@ -25,8 +30,11 @@ struct lsx_lu {
     int64_t* dinfo = nullptr;    // device: [0] info, [1] error flags
     int64_t* hinfo = nullptr;    // pinned host mirror of dinfo
     uint64_t* slots = nullptr;   // panel exchange records
-    int64_t* plist = nullptr;    // permutation lists: inner [0, 4*NBO), outer [4*NBO, 8*NBO)
-    int* pcount = nullptr;       // [0] inner T, [1] outer T
+    int64_t* plist = nullptr;    // permutation lists: inner [0, 4*NBO), outer by block parity
+                                 // [4*NBO, 8*NBO) and [8*NBO, 12*NBO)
+    int* pcount = nullptr;       // [0] inner T, [1], [2] outer T by block parity
+    int* pinv = nullptr;         // per outer block: inverse of the row map after that block
+                                 // (the deferred row interchanges of the L columns)
     uint32_t* tflags = nullptr;  // triangular-solve block flags (2 * nblk)
     double* DL = nullptr;        // inverses of the 64 x 64 diagonal blocks of L (nblk x 64 x 64)
     double* DU = nullptr;        // ... and of U

@@ -131,11 +131,11 @@ static void gemm_f64(lsx_lu* lu, const double* A, int64_t lda, const double* B, 
     if (M <= 0 || N <= 0 || K <= 0) return;
     const int64_t T = cdiv(M, GBM) * cdiv(N, GBN);
     const int64_t grid = 8 * cdiv(T, 8);
-    const int pi = lu->prof.begin(lu->stream, PK_GEMM, 2.0 * M * N * K,
+    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K,
                                   sizeof(double) * (2.0 * M * N + (double)M * K + (double)K * N));
-    hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)grid), dim3(256), 0, lu->stream, A, lda, B, ldb, C, ldc,
+    hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)grid), dim3(256), 0, lu->cur, A, lda, B, ldb, C, ldc,
                        (int)M, (int)N, (int)K, mode);
-    lu->prof.end(lu->stream, pi);
+    lu->prof.end(lu->cur, pi);
 }
 
 int lu64_getrf(lsx_lu* lu) {

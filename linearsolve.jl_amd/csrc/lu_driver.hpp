@@ -26,16 +26,16 @@ static void launch_panel_t(lsx_lu* lu, T* F, int64_t ld, int64_t j0, int W) {
     a.seq = lu->seq;
     a.timeout = lu->ctx->wall_khz * 5000;  // 5 s
     lu->seq += (uint32_t)W + 1u;
-    const int pi = lu->prof.begin(lu->stream, PK_PANEL, (double)m_rows * W * W,
+    const int pi = lu->prof.begin(lu->cur, PK_PANEL, (double)m_rows * W * W,
                                   2.0 * sizeof(T) * (double)m_rows * W);
     if (m_rows <= 256) {
         a.P = 1;
-        hipLaunchKernelGGL((k_panel<T, 256>), dim3(1), dim3(256), 0, lu->stream, a);
+        hipLaunchKernelGGL((k_panel<T, 256>), dim3(1), dim3(256), 0, lu->cur, a);
     } else {
         a.P = (int)cdiv(m_rows, 512);
-        hipLaunchKernelGGL((k_panel<T, 512>), dim3(a.P), dim3(512), 0, lu->stream, a);
+        hipLaunchKernelGGL((k_panel<T, 512>), dim3(a.P), dim3(512), 0, lu->cur, a);
     }
-    lu->prof.end(lu->stream, pi);
+    lu->prof.end(lu->cur, pi);
 }
 
 template <typename T>
@@ -43,39 +43,48 @@ static void launch_laswp_t(lsx_lu* lu, T* F, int64_t ld, int64_t k0, int kb, con
                            const int* count, int64_t c0, int64_t c1, int trsm) {
     if (c1 <= c0) return;
     const int64_t nb = cdiv(c1 - c0, LCW);
-    const int pi = lu->prof.begin(lu->stream, PK_LASWP, trsm ? (double)kb * kb * (c1 - c0) : 0.0,
+    const int pi = lu->prof.begin(lu->cur, PK_LASWP, trsm ? (double)kb * kb * (c1 - c0) : 0.0,
                                   sizeof(T) * 4.0 * kb * (c1 - c0));
-    hipLaunchKernelGGL(k_laswp_trsm<T>, dim3((unsigned)nb), dim3(256), 0, lu->stream, F, ld, k0, kb, list, count,
+    hipLaunchKernelGGL(k_laswp_trsm<T>, dim3((unsigned)nb), dim3(256), 0, lu->cur, F, ld, k0, kb, list, count,
                        c0, c1, trsm);
-    lu->prof.end(lu->stream, pi);
+    lu->prof.end(lu->cur, pi);
 }
 
 // Factors F (n x n, ld) in place; pivots into lu->ipiv, row map into lu->perm, diagonal-block
 // inverses into DL / DU.  gemm_upd does the inner and trailing updates (mode 0: C -= A B),
-// gemm_trsm the products of the blocked TRSM (mode 1: C = A B in place, mode 0).
+// gemm_trsm the products of the blocked TRSM (mode 1: C = A B in place, mode 0); both launch on
+// lu->cur.
+//
+// Lookahead: the trailing update of outer block b first updates the columns of block b + 1 on
+// lu->stream, then block b + 1 is factored on lu->stream2 while lu->stream updates the remaining
+// columns.  The row interchanges of a block are applied right of it at once and, when a column
+// segment fits in LDS, left of it (the L columns) in one pass at the end.
 template <typename T, typename GU, typename GT>
 int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT gemm_trsm) {
     lsx_ctx* ctx = lu->ctx;
     const int64_t n = lu->n;
+    const hipStream_t s0 = lu->stream, s1 = lu->stream2;
+    const bool defer = lu->pinv != nullptr && (n - NBO) * (int64_t)sizeof(T) <= LGMAX_BYTES;
+    lu->cur = s0;
     if (lu->seq > 0xF0000000u) {
-        LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, lu->stream));
+        LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, s0));
         lu->seq = 1;
     }
-    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo, 0, 2 * sizeof(int64_t), lu->stream));
-    hipLaunchKernelGGL(k_iota, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->stream, lu->perm, n);
+    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo, 0, 2 * sizeof(int64_t), s0));
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, s0, lu->perm, n);
     int64_t* il = lu->plist;
-    int64_t* ol = lu->plist + 4 * NBO;
     int* ic = lu->pcount;
-    int* oc = lu->pcount + 1;
-    for (int64_t jo = 0; jo < n; jo += NBO) {
-        const int jbo = (int)std::min<int64_t>(NBO, n - jo);
-        // inner panels of PW columns inside the outer block
+
+    // factorization of outer block [jo, jo + jbo) (its columns already updated): inner panels,
+    // their interchanges and updates inside the block, diagonal-block inverses, the block's
+    // interchange list (parity par) and the row map
+    auto panel_block = [&](int64_t jo, int jbo, int par) {
         for (int64_t ji = jo; ji < jo + jbo; ji += PW) {
             const int W = (int)std::min<int64_t>(PW, jo + jbo - ji);
             launch_panel_t<T>(lu, F, ld, ji, W);
             const bool left = ji > jo, right = ji + W < jo + jbo;
             if (left || right) {
-                hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->stream, lu->ipiv, ji, W, il, ic,
+                hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, ji, W, il, ic,
                                    (int*)nullptr);
                 if (left) launch_laswp_t<T>(lu, F, ld, ji, W, il, ic, jo, ji, 0);
                 if (right) {
@@ -85,34 +94,65 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
                 }
             }
         }
-        // inverses of the diagonal 64-blocks of this outer block (final from here on)
         const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
-        {
-            const int pi = lu->prof.begin(lu->stream, PK_OTHER, 0.0, 0.0);
-            hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(DB), 0, lu->stream, F, ld, n, b0, DL, DU);
-            lu->prof.end(lu->stream, pi);
+        const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
+        hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(DB), 0, lu->cur, F, ld, n, b0, DL, DU);
+        lu->prof.end(lu->cur, pi);
+        hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
+                           lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);
+        if (defer)
+            hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
+                               lu->pinv + (jo / NBO) * n, n);
+    };
+    // interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 of block [jo, jo + jbo) on columns [c0, c1)
+    auto update = [&](int64_t jo, int jbo, int par, int64_t c0, int64_t c1) {
+        if (c1 <= c0) return;
+        const int64_t r1 = jo + jbo, nc = c1 - c0;
+        launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, c0, c1, 0);
+        // L11 = [L1 0; L2 L3] (64-blocks): X1 = L1^{-1} A12[0:64];  A12[64:] -= L2 X1;  X2 = L3^{-1} A12[64:]
+        T* A12 = F + c0 * ld + jo;
+        const int64_t b0 = jo / DB;
+        const int64_t h1 = std::min<int64_t>(DB, jbo), h2 = jbo - h1;
+        gemm_trsm(DL + b0 * DB * DB, DB, A12, ld, A12, ld, h1, nc, h1, 1);
+        if (h2 > 0) {
+            gemm_trsm(F + jo * ld + jo + DB, ld, A12, ld, A12 + DB, ld, h2, nc, DB, 0);
+            gemm_trsm(DL + (b0 + 1) * DB * DB, DB, A12 + DB, ld, A12 + DB, ld, h2, nc, h2, 1);
         }
-        hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->stream, lu->ipiv, jo, jbo, ol, oc, lu->perm);
-        if (jo > 0) launch_laswp_t<T>(lu, F, ld, jo, jbo, ol, oc, 0, jo, 0);
-        if (jo + jbo < n) {
-            const int64_t c1 = jo + jbo, nr = n - c1;
-            launch_laswp_t<T>(lu, F, ld, jo, jbo, ol, oc, c1, n, 0);
-            // U12 = L11^{-1} A12 with L11 = [L1 0; L2 L3] (64-blocks):
-            //   X1 = L1^{-1} A12[0:64];  A12[64:] -= L2 X1;  X2 = L3^{-1} A12[64:]
-            T* A12 = F + c1 * ld + jo;
-            const int64_t h1 = std::min<int64_t>(DB, jbo), h2 = jbo - h1;
-            gemm_trsm(DL + b0 * DB * DB, DB, A12, ld, A12, ld, h1, nr, h1, 1);
-            if (h2 > 0) {
-                gemm_trsm(F + jo * ld + jo + DB, ld, A12, ld, A12 + DB, ld, h2, nr, DB, 0);
-                gemm_trsm(DL + (b0 + 1) * DB * DB, DB, A12 + DB, ld, A12 + DB, ld, h2, nr, h2, 1);
-            }
-            // trailing update A22 -= L21 U12
-            gemm_upd(F + jo * ld + c1, ld, A12, ld, F + c1 * ld + c1, ld, nr, nr, jbo, 0);
-        }
+        gemm_upd(F + jo * ld + r1, ld, A12, ld, F + c0 * ld + r1, ld, n - r1, nc, jbo, 0);
+    };
+
+    panel_block(0, (int)std::min<int64_t>(NBO, n), 0);
+    int par = 0;
+    for (int64_t jo = 0; jo < n; jo += NBO, par ^= 1) {
+        const int jbo = (int)std::min<int64_t>(NBO, n - jo);
+        const int64_t c1 = jo + jbo;
+        lu->cur = s0;
+        if (!defer && jo > 0)
+            launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, 0, jo, 0);
+        if (c1 >= n) break;
+        const int jbn = (int)std::min<int64_t>(NBO, n - c1);
+        update(jo, jbo, par, c1, c1 + jbn);
+        LSX_HIP(ctx, hipEventRecord(lu->ev_next, s0));
+        LSX_HIP(ctx, hipStreamWaitEvent(s1, lu->ev_next, 0));
+        lu->cur = s1;
+        panel_block(c1, jbn, par ^ 1);
+        LSX_HIP(ctx, hipEventRecord(lu->ev_panel, s1));
+        lu->cur = s0;
+        update(jo, jbo, par, c1 + jbn, n);
+        LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_panel, 0));
+    }
+    lu->cur = s0;
+    if (defer && n > NBO) {
+        const int64_t ncols = (cdiv(n, NBO) - 1) * NBO;
+        const size_t lds = sizeof(T) * (size_t)(n - NBO);
+        const int pi = lu->prof.begin(s0, PK_LASWP, 0.0, 2.0 * sizeof(T) * (double)n * n / 2);
+        hipLaunchKernelGGL(k_left_gather<T>, dim3((unsigned)ncols), dim3(256), lds, s0, F, ld, n,
+                           (const int*)lu->perm, (const int*)lu->pinv);
+        lu->prof.end(s0, pi);
     }
     LSX_HIP(ctx, hipGetLastError());
-    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo, lu->dinfo, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, lu->stream));
-    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo, lu->dinfo, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s0));
+    LSX_HIP(ctx, hipStreamSynchronize(s0));
     lu->prof.collect();
     if (lu->hinfo[1] != 0)
         return fail(ctx, LSX_ETIMEOUT, "getrf: a panel exchange wait exceeded its bound (flags %lld)",

@@ -305,6 +305,34 @@ static __global__ __launch_bounds__(256) void k_perm_build(const int64_t* __rest
     }
 }
 
+// Inverse of a row map: inv[perm[r]] = r.
+static __global__ void k_inv_perm(const int* __restrict perm, int* __restrict inv, int64_t n) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) inv[perm[r]] = (int)r;
+}
+
+// Deferred row interchanges of the L columns.  While the matrix is factored, the interchanges of
+// an outer block are applied to the columns right of it only; column c of outer block b keeps
+// its rows [r0, n) (r0 = end of block b) in the row order the matrix had after block b.  At the
+// end, row i of that segment takes the stored row inv_b[perm[i]]: perm is the final row map and
+// inv_b the inverse of the row map after block b.  One workgroup per column, the segment staged
+// in LDS (dynamic, (n - NBO) elements at most).
+template <typename T>
+__global__ __launch_bounds__(256) void k_left_gather(T* __restrict F, int64_t ld, int64_t n,
+                                                     const int* __restrict perm, const int* __restrict pinv) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    T* s = reinterpret_cast<T*>(s_raw);
+    const int64_t c = blockIdx.x;
+    const int64_t b = c / NBO;
+    const int64_t r0 = (b + 1) * NBO;
+    if (r0 >= n) return;
+    T* col = F + c * ld;
+    for (int64_t i = r0 + threadIdx.x; i < n; i += 256) s[i - r0] = col[i];
+    __syncthreads();
+    const int* inv = pinv + b * n;
+    for (int64_t i = r0 + threadIdx.x; i < n; i += 256) col[i] = s[inv[perm[i]] - r0];
+}
+
 static __global__ void k_iota(int* __restrict p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = (int)i;

@@ -409,22 +409,22 @@ static void gemm_f32(lsx_lu* lu, const float* A, int64_t lda, const float* B, in
                      int64_t M, int64_t N, int64_t K, int mode) {
     if (M <= 0 || N <= 0 || K <= 0) return;
     const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
-    const int pi = lu->prof.begin(lu->stream, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
-    hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->stream, A, lda, B, ldb, C, ldc,
+    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->cur, A, lda, B, ldb, C, ldc,
                        (int)M, (int)N, (int)K, mode);
-    lu->prof.end(lu->stream, pi);
+    lu->prof.end(lu->cur, pi);
 }
 
 static void gemm_bf16(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                       int64_t M, int64_t N, int64_t K) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    float* sc = ms(lu)->scale;
-    hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, lu->stream, B, ldb, (int)K, (int)N, sc);
+    float* sc = ms(lu)->scale + (lu->cur == lu->stream2 ? 1 : 0);  // one per getrf stream
+    hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
     const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
-    const int pi = lu->prof.begin(lu->stream, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
-    hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->stream, A, lda, B, ldb, C,
+    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->cur, A, lda, B, ldb, C,
                        ldc, (int)M, (int)N, (int)K, (const float*)sc);
-    lu->prof.end(lu->stream, pi);
+    lu->prof.end(lu->cur, pi);
 }
 
 int mixed_getrf(lsx_lu* lu) {
