@@ -116,6 +116,14 @@ def lib():
             if _lib is None:
                 if not os.path.exists(LIB_PATH):
                     raise ImportError(f"{LIB_PATH} is not built: the gfx950 HIP library is required")
+                # One HIP runtime per process.  PyTorch-ROCm ships its own libamdhip64.so.7 (same
+                # soname as the system one liblsx.so is linked against); whichever is loaded first
+                # serves both, and torch fails to initialise on a runtime other than its own.  So
+                # when torch is installed it is loaded before liblsx.so.
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
                 L = ctypes.CDLL(LIB_PATH)
                 for name, res, args in SIGNATURES:
                     f = getattr(L, name)
