@@ -1,6 +1,7 @@
 // api.hip — the C-ABI entry points of liblsx.so (declared in include/lsx.h).
 #include <algorithm>
 #include <new>
+#include <vector>
 
 #include "lu.hpp"
 
@@ -23,7 +24,27 @@ int lu64_alloc(lsx_lu* lu) {
     lu->ld = pad_ld(n);
     const int nblk = (int)cdiv(n, DB);
     LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream, hipStreamNonBlocking));
-    LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
+    // getrf lookahead streams.  The panel workgroups wait on each other, so they get CUs of their
+    // own: 4 per XCD (32 in all, enough for the 512-row workgroups of n <= 16384) and the
+    // trailing updates the rest.  The CU set {per*x + (x + 8j) mod per} puts 4 CUs on every XCD
+    // whether CU mask bits are numbered XCD by XCD or round-robin over the XCDs.
+    const int per = ctx->num_cus / 8;
+    if (ctx->num_cus % 8 == 0 && per % 8 == 0 && per >= 32 && cdiv(n, 512) <= 32) {
+        const int words = (ctx->num_cus + 31) / 32;
+        std::vector<uint32_t> ms(words, 0u), mc(words, 0u);
+        for (int x = 0; x < 8; ++x)
+            for (int j = 0; j < 4; ++j) {
+                const int i = per * x + (x + 8 * j) % per;
+                ms[i / 32] |= 1u << (i % 32);
+            }
+        for (int i = 0; i < ctx->num_cus; ++i)
+            if (!((ms[i / 32] >> (i % 32)) & 1u)) mc[i / 32] |= 1u << (i % 32);
+        LSX_HIP(ctx, hipExtStreamCreateWithCUMask(&lu->stream2, (uint32_t)words, ms.data()));
+        LSX_HIP(ctx, hipExtStreamCreateWithCUMask(&lu->stream_upd, (uint32_t)words, mc.data()));
+    } else {
+        LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
+    }
+    LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_start, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_next, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_panel, hipEventDisableTiming));
     lu->cur = lu->stream;
@@ -68,9 +89,12 @@ void lu64_free(lsx_lu* lu) {
     (void)hipFree(lu->wb);
     (void)hipFree(lu->wy);
     if (lu->stream2) (void)hipStreamSynchronize(lu->stream2);
+    if (lu->stream_upd) (void)hipStreamSynchronize(lu->stream_upd);
+    if (lu->ev_start) (void)hipEventDestroy(lu->ev_start);
     if (lu->ev_next) (void)hipEventDestroy(lu->ev_next);
     if (lu->ev_panel) (void)hipEventDestroy(lu->ev_panel);
     if (lu->stream2) (void)hipStreamDestroy(lu->stream2);
+    if (lu->stream_upd) (void)hipStreamDestroy(lu->stream_upd);
     if (lu->stream) (void)hipStreamDestroy(lu->stream);
 }
 }  // namespace lsx

@@ -20,6 +20,9 @@ struct lsx_lu {
     int64_t ld = 0;
     hipStream_t stream = nullptr;   // trailing updates and everything outside getrf
     hipStream_t stream2 = nullptr;  // getrf lookahead: panel factorization of the next block
+    hipStream_t stream_upd = nullptr;  // getrf trailing updates on the CUs stream2 does not use
+                                       // (null: no CU partition, updates run on stream)
+    hipEvent_t ev_start = nullptr;  // getrf input ready on stream
     hipStream_t cur = nullptr;      // stream the launch helpers use at the moment
     hipEvent_t ev_next = nullptr;   // next block's columns updated (stream -> stream2)
     hipEvent_t ev_panel = nullptr;  // next block's panel factored (stream2 -> stream)

@@ -63,8 +63,12 @@ template <typename T, typename GU, typename GT>
 int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT gemm_trsm) {
     lsx_ctx* ctx = lu->ctx;
     const int64_t n = lu->n;
-    const hipStream_t s0 = lu->stream, s1 = lu->stream2;
+    const hipStream_t s0 = lu->stream_upd ? lu->stream_upd : lu->stream, s1 = lu->stream2;
     const bool defer = lu->pinv != nullptr && (n - NBO) * (int64_t)sizeof(T) <= LGMAX_BYTES;
+    if (s0 != lu->stream) {  // the input (copied on lu->stream) before anything on s0
+        LSX_HIP(ctx, hipEventRecord(lu->ev_start, lu->stream));
+        LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_start, 0));
+    }
     lu->cur = s0;
     if (lu->seq > 0xF0000000u) {
         LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, s0));
@@ -153,6 +157,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
     LSX_HIP(ctx, hipGetLastError());
     LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo, lu->dinfo, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s0));
     LSX_HIP(ctx, hipStreamSynchronize(s0));
+    lu->cur = lu->stream;
     lu->prof.collect();
     if (lu->hinfo[1] != 0)
         return fail(ctx, LSX_ETIMEOUT, "getrf: a panel exchange wait exceeded its bound (flags %lld)",

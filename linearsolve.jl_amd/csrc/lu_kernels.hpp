@@ -33,8 +33,16 @@ struct PanelArgs {
     int64_t timeout;
 };
 
-// Words (32-bit) per element of an exchange record.
-template <typename T> struct Words { static constexpr int W = sizeof(T) / 4; };
+// Words (32-bit) per element of an exchange record, and word i of an element.
+template <typename T> struct Words;
+template <> struct Words<double> {
+    static constexpr int W = 2;
+    __device__ static uint32_t word(double x, int i) { return (uint32_t)(i ? __double2hiint(x) : __double2loint(x)); }
+};
+template <> struct Words<float> {
+    static constexpr int W = 1;
+    __device__ static uint32_t word(float x, int) { return __float_as_uint(x); }
+};
 // Exchange record (32-bit words): [0,1] key (lo, hi), [2] logical row, [3] pad, [4, ...) the
 // candidate row in the panel's register layout.  16-byte multiple, so the row data is aligned.
 template <typename T> constexpr int rec_words() { return 4 + PW * Words<T>::W; }
@@ -46,14 +54,31 @@ template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
 template <int CTRL> __device__ __forceinline__ uint64_t dpp64(uint64_t x) {
     return ((uint64_t)dpp32<CTRL>((uint32_t)(x >> 32)) << 32) | dpp32<CTRL>((uint32_t)x);
 }
+// Value of the lane 16 (32) apart across rows of 16 (halves of 32), by v_permlane16_swap /
+// v_permlane32_swap of x with itself: the swap gives each lane its own value in one result and the
+// partner's in the other.
+__device__ __forceinline__ uint32_t xrow16(uint32_t x) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return ((threadIdx.x >> 4) & 1) ? r[0] : r[1];
+}
+__device__ __forceinline__ uint32_t xrow32(uint32_t x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return ((threadIdx.x >> 5) & 1) ? r[0] : r[1];
+}
+__device__ __forceinline__ uint64_t xrow16_64(uint64_t x) {
+    return ((uint64_t)xrow16((uint32_t)(x >> 32)) << 32) | xrow16((uint32_t)x);
+}
+__device__ __forceinline__ uint64_t xrow32_64(uint64_t x) {
+    return ((uint64_t)xrow32((uint32_t)(x >> 32)) << 32) | xrow32((uint32_t)x);
+}
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
     uint64_t y;
     y = dpp64<0xB1>(x);  x = y > x ? y : x;   // quad_perm [1,0,3,2]: lane ^ 1
     y = dpp64<0x4E>(x);  x = y > x ? y : x;   // quad_perm [2,3,0,1]: lane ^ 2
     y = dpp64<0x141>(x); x = y > x ? y : x;   // row_half_mirror: pairs the two quads of each 8
     y = dpp64<0x140>(x); x = y > x ? y : x;   // row_mirror: pairs the two halves of each 16
-    y = __shfl_xor((unsigned long long)x, 16); x = y > x ? y : x;
-    y = __shfl_xor((unsigned long long)x, 32); x = y > x ? y : x;
+    y = xrow16_64(x);    x = y > x ? y : x;
+    y = xrow32_64(x);    x = y > x ? y : x;
     return x;
 }
 __device__ __forceinline__ int wave_min_i32(int x) {
@@ -62,8 +87,8 @@ __device__ __forceinline__ int wave_min_i32(int x) {
     y = (int)dpp32<0x4E>((uint32_t)x);  x = y < x ? y : x;
     y = (int)dpp32<0x141>((uint32_t)x); x = y < x ? y : x;
     y = (int)dpp32<0x140>((uint32_t)x); x = y < x ? y : x;
-    y = __shfl_xor(x, 16); x = y < x ? y : x;
-    y = __shfl_xor(x, 32); x = y < x ? y : x;
+    y = (int)xrow16((uint32_t)x);       x = y < x ? y : x;
+    y = (int)xrow32((uint32_t)x);       x = y < x ? y : x;
     return x;
 }
 // Lane holding the best (u, lr) pair of the wave: largest u, then smallest logical row lr.
@@ -93,11 +118,15 @@ template <typename T> __device__ __forceinline__ uint64_t pivot_key(T x) {
 // finished; at the end every row is written to its logical position.  Ties of the pivot search
 // are broken by the logical index, as an explicit row interchange would.
 // P workgroups of BS threads cover rows [j0, m); with P > 1 each column exchanges one record
-// per workgroup (its best candidate and that row's data) as flag-in-word messages.
+// per workgroup (its best candidate and that row's data) as flag-in-word messages.  There the
+// rank-1 update of column c is deferred into column c + 1: it is applied at once to the new
+// pivot column (needed for the pivot search) and to the published candidate row, and to the
+// rest of the row while the exchange is in flight.
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
     constexpr int NW = BS / 64;
     constexpr int RW = rec_words<T>();
+    constexpr int WPE = Words<T>::W;
     const int tid = threadIdx.x, w = blockIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r = a.j0 + (int64_t)w * BS + tid;
     const bool valid = r < a.m;
@@ -111,37 +140,132 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
     if (w == 0 && tid == 0) info0 = a.dinfo[0];
 
     __shared__ __align__(16) uint32_t s_cand[2][NW][RW];
+    __shared__ double s_candl[2][NW];
     __shared__ __align__(16) uint32_t s_rec[2][PMAX * RW];
 
+    // interchange of column k with pivot row grow (relabelling), pivots and info
+    auto pivot_bookkeeping = [&](int k, int grow, T pv) {
+        if (lr == grow) {
+            live = false;
+            lr = k;
+        } else if (lr == k) {
+            lr = grow;
+        }
+        if (w == 0 && tid == 0) {
+            a.ipiv[k] = (int64_t)grow + 1;
+            if (pv == T(0) && info0 == 0) {
+                info0 = k + 1;
+                a.dinfo[0] = info0;
+            }
+        }
+    };
+    auto scale = [&](T x, T pv) -> T {
+        if (live && pv != T(0)) {
+            if (fabs(pv) >= std::numeric_limits<T>::min()) x *= (T(1) / pv);
+            else x /= pv;
+        }
+        return x;
+    };
+
     const int P = a.P;
+    if (P == 1) {
+        // one workgroup: everything through LDS, update right after each pivot
 #pragma unroll 1
-    for (int c = 0; c < a.W; ++c) {
-        const int par = c & 1;
-        const int k = (int)(a.j0 + c);
-        // wave candidate -> LDS
-        const uint64_t u = live ? pivot_key(v[0]) : 0;
-        const int wl = wave_argmax(u, lr);
-        if (lane == wl) {
-            uint32_t* d = s_cand[par][wave];
-            *reinterpret_cast<uint64_t*>(d) = u;
-            d[2] = (uint32_t)lr;
-            T* dv = reinterpret_cast<T*>(d + 4);
+        for (int c = 0; c < a.W; ++c) {
+            const int par = c & 1;
+            const int k = (int)(a.j0 + c);
+            const uint64_t u = live ? pivot_key(v[0]) : 0;
+            const int wl = wave_argmax(u, lr);
+            if (lane == wl) {
+                uint32_t* d = s_cand[par][wave];
+                *reinterpret_cast<uint64_t*>(d) = u;
+                d[2] = (uint32_t)lr;
+                T* dv = reinterpret_cast<T*>(d + 4);
 #pragma unroll
-            for (int q = 0; q < PW; ++q) dv[q] = v[q];
+                for (int q = 0; q < PW; ++q) dv[q] = v[q];
+            }
+            __syncthreads();
+            int wi;
+            {
+                const uint64_t cu = lane < NW ? *reinterpret_cast<const uint64_t*>(s_cand[par][lane]) : 0;
+                const int cl = lane < NW ? (int)s_cand[par][lane][2] : INT_MAX;
+                wi = wave_argmax(cu, cl);
+            }
+            const uint32_t* win = s_cand[par][wi];
+            const T* prow = reinterpret_cast<const T*>(win + 4);
+            const int grow = (int)win[2];
+            const T pv = prow[0];
+            pivot_bookkeeping(k, grow, pv);
+            const T l = scale(v[0], pv);
+            const int lim = live ? PW - 1 - c : 0;  // slots 1..lim hold columns still to eliminate
+            // in groups of 8 slots so that the pivot-row LDS reads are not all hoisted at once
+#pragma unroll
+            for (int p0 = 0; p0 < PW; p0 += 8) {
+#pragma unroll
+                for (int p = p0; p < p0 + 8 && p < PW - 1; ++p) {
+                    const T nv = v[p + 1];
+                    v[p] = (p < lim) ? fma(-l, prow[p + 1], nv) : nv;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            v[PW - 1] = l;
         }
-        __syncthreads();
-        // workgroup candidate (every wave reduces the NW candidates itself)
-        int wi;
-        {
-            const uint64_t cu = lane < NW ? *reinterpret_cast<const uint64_t*>(s_cand[par][lane]) : 0;
-            const int cl = lane < NW ? (int)s_cand[par][lane][2] : INT_MAX;
-            wi = wave_argmax(cu, cl);
-        }
-        const uint32_t* win = s_cand[par][wi];
-        if (P > 1) {
+    } else {
+        // pending update (column c - 1): pend = this row takes it, multiplier lp, pivot row pprev
+        bool pend = false;
+        T lp = T(0);
+        const T* pprev = nullptr;
+#pragma unroll 1
+        for (int c = 0; c < a.W; ++c) {
+            const int par = c & 1;
+            const int k = (int)(a.j0 + c);
+            const int plim = PW - c;  // the pending update covers slots [0, plim)
+            if (pend) v[0] = fma(-lp, pprev[1], v[0]);
+            const uint64_t u = live ? pivot_key(v[0]) : 0;
+            const int wl = wave_argmax(u, lr);
+            if (lane == wl) {  // raw row (slot 0 already current) and its pending multiplier
+                uint32_t* d = s_cand[par][wave];
+                *reinterpret_cast<uint64_t*>(d) = u;
+                d[2] = (uint32_t)lr;
+                T* dv = reinterpret_cast<T*>(d + 4);
+#pragma unroll
+                for (int q = 0; q < PW; ++q) dv[q] = v[q];
+                s_candl[par][wave] = (double)lp;
+            }
+            __syncthreads();
+            int wi;
+            {
+                const uint64_t cu = lane < NW ? *reinterpret_cast<const uint64_t*>(s_cand[par][lane]) : 0;
+                const int cl = lane < NW ? (int)s_cand[par][lane][2] : INT_MAX;
+                wi = wave_argmax(cu, cl);
+            }
             const uint32_t seq = a.seq + (uint32_t)c + 1u;
             uint64_t* g = a.slots + (size_t)par * (PMAX + 1) * RECS;
-            if (tid < RW) st_relaxed_u64(g + (size_t)w * RECS + tid, ll_pack(win[tid], seq));
+            if (tid < RW) {
+                // word tid of this workgroup's record, the candidate row brought up to date
+                const uint32_t* win = s_cand[par][wi];
+                uint32_t word = win[tid];
+                if (tid >= 4 && pprev) {
+                    const int q = (tid - 4) / WPE;
+                    if (q > 0 && q < plim) {
+                        const T x = fma(-(T)s_candl[par][wi], pprev[q + 1], reinterpret_cast<const T*>(win + 4)[q]);
+                        word = Words<T>::word(x, (tid - 4) % WPE);
+                    }
+                }
+                st_relaxed_u64(g + (size_t)w * RECS + tid, ll_pack(word, seq));
+            }
+            // the rest of the pending update, and the rotation by one slot, while the records
+            // travel (slot 0, column c, is kept in c0 until the pivot is known)
+            const T c0 = v[0];
+#pragma unroll
+            for (int p0 = 1; p0 < PW; p0 += 8) {
+#pragma unroll
+                for (int p = p0; p < p0 + 8 && p < PW; ++p) {
+                    const T nv = v[p];
+                    v[p - 1] = (pend && p < plim) ? fma(-lp, pprev[p + 1], nv) : nv;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
             // every thread reads a share of the words of all P records: all of its loads are
             // issued at once, then the ones not yet current are re-polled
             constexpr int MAXW = (PMAX * RW + BS - 1) / BS;
@@ -178,53 +302,28 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
             int bl = INT_MAX, bp = 0;
 #pragma unroll
             for (int h = 0; h < PMAX / 64; ++h) {
-                const int p = lane + 64 * h;
-                if (p < P) {
-                    const uint32_t* rp = s_rec[par] + p * RW;
+                const int pidx = lane + 64 * h;
+                if (pidx < P) {
+                    const uint32_t* rp = s_rec[par] + pidx * RW;
                     const uint64_t pu = *reinterpret_cast<const uint64_t*>(rp);
                     const int pl = (int)rp[2];
-                    if (pu > bu || (pu == bu && pl < bl)) { bu = pu; bl = pl; bp = p; }
+                    if (pu > bu || (pu == bu && pl < bl)) { bu = pu; bl = pl; bp = pidx; }
                 }
             }
             const int wlane = wave_argmax(bu, bl);
             const int pw = __shfl(bp, wlane);
-            win = s_rec[par] + pw * RW;
+            const uint32_t* win = s_rec[par] + pw * RW;
+            const T* prow = reinterpret_cast<const T*>(win + 4);
+            const int grow = (int)win[2];
+            const T pv = prow[0];
+            pivot_bookkeeping(k, grow, pv);
+            const T l = scale(c0, pv);
+            v[PW - 1] = l;
+            pend = live;
+            lp = l;
+            pprev = prow;
         }
-        const T* prow = reinterpret_cast<const T*>(win + 4);
-        const int grow = (int)win[2];
-        const T pv = prow[0];
-        // interchange rows k and grow by relabelling
-        if (lr == grow) {
-            live = false;
-            lr = k;
-        } else if (lr == k) {
-            lr = grow;
-        }
-        if (w == 0 && tid == 0) {
-            a.ipiv[k] = (int64_t)grow + 1;
-            if (pv == T(0) && info0 == 0) {
-                info0 = k + 1;
-                a.dinfo[0] = info0;
-            }
-        }
-        // scale, rank-1 update of the remaining columns, rotate by one slot
-        T l = v[0];
-        if (live && pv != T(0)) {
-            if (fabs(pv) >= std::numeric_limits<T>::min()) l *= (T(1) / pv);
-            else l /= pv;
-        }
-        const int lim = live ? PW - 1 - c : 0;  // slots 1..lim hold columns still to eliminate
-        // in groups of 8 slots so that the pivot-row LDS reads are not all hoisted at once
-#pragma unroll
-        for (int p0 = 0; p0 < PW; p0 += 8) {
-#pragma unroll
-            for (int p = p0; p < p0 + 8 && p < PW - 1; ++p) {
-                const T nv = v[p + 1];
-                v[p] = (p < lim) ? fma(-l, prow[p + 1], nv) : nv;
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        v[PW - 1] = l;
+        // the update of the last column touches only columns >= W (padding): nothing pending
     }
     if (valid) {
         // after W rotations slot p holds logical column (p + W) mod PW; the row goes to its
