@@ -186,6 +186,7 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     tg = torch.Generator(device="cuda")
     tg.manual_seed(44)
     A = torch.randn(total, dtype=torch.float64, device="cuda", generator=tg)
+    log("batched: J generated")
     scale = torch.repeat_interleave(torch.from_numpy(-gam / np.sqrt(sizes)).cuda(), torch.from_numpy(nn).cuda())
     A.mul_(scale)
     del scale
@@ -194,6 +195,7 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     b = torch.randn(totv, dtype=torch.float64, device="cuda", generator=tg)
     x = torch.empty_like(b)
     torch.cuda.synchronize()
+    log("batched: systems ready, creating the batch handle")
     hb = ctypes.c_void_p()
     ctx.check(L.lsx_batch_create(ctx.handle, count, sizes.ctypes.data, _abi.LSX_F64, ctypes.byref(hb)),
               "lsx_batch_create")
@@ -204,8 +206,10 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
                                                   info.ctypes.data, None), "lsx_batch_factor_solve")
 
     batched()
+    log("batched: first launch done")
     tb, nfail = timed(batched, reps)
     L.lsx_batch_destroy(hb)
+    log(f"batched: {reps} timed launches done, {tb * 1e3:.1f} ms")
     # backward error on a sample of systems (host FP64)
     rs = np.random.default_rng(7)
     pick = rs.choice(count, size=min(200, count), replace=False)
@@ -217,6 +221,7 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
         xi = x[voff[i]:voff[i] + n].cpu().numpy()
         r = bi - Ai @ xi
         worst = max(worst, np.abs(r).max() / (np.abs(Ai).sum(axis=1).max() * np.abs(xi).max() + np.abs(bi).max()))
+    log(f"batched: sampled backward error {worst:.2e}")
     # per-system loop of the plain FP64 GPU LU (handles made beforehand, outside the timing)
     loop_idx = rs.choice(count, size=min(loop_sample, count), replace=False)
     handles = {}
@@ -227,6 +232,7 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
             ctx.check(L.lsx_lu_create(ctx.handle, n, _abi.LSX_F64, ctypes.byref(h)), "lsx_lu_create")
             handles[n] = h
     torch.cuda.synchronize()
+    log(f"batched: {len(handles)} per-size LU handles made, timing the per-system loop")
     t0 = time.perf_counter()
     for i in loop_idx:
         n = int(sizes[i])

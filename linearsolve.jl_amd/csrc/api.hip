@@ -28,8 +28,11 @@ int lu64_alloc(lsx_lu* lu) {
     // own: 4 per XCD (32 in all, enough for the 512-row workgroups of n <= 16384) and the
     // trailing updates the rest.  The CU set {per*x + (x + 8j) mod per} puts 4 CUs on every XCD
     // whether CU mask bits are numbered XCD by XCD or round-robin over the XCDs.
+    // (only for n >= 2048: below that the lookahead gains nothing, and a CU-masked stream takes a
+    // hardware queue of its own, of which a process has few)
     const int per = ctx->num_cus / 8;
-    if (ctx->num_cus % 8 == 0 && per % 8 == 0 && per >= 32 && cdiv(n, 512) <= 32) {
+    bool masked = false;
+    if (n >= 2048 && ctx->num_cus % 8 == 0 && per % 8 == 0 && per >= 32 && cdiv(n, 512) <= 32) {
         const int words = (ctx->num_cus + 31) / 32;
         std::vector<uint32_t> ms(words, 0u), mc(words, 0u);
         for (int x = 0; x < 8; ++x)
@@ -39,11 +42,16 @@ int lu64_alloc(lsx_lu* lu) {
             }
         for (int i = 0; i < ctx->num_cus; ++i)
             if (!((ms[i / 32] >> (i % 32)) & 1u)) mc[i / 32] |= 1u << (i % 32);
-        LSX_HIP(ctx, hipExtStreamCreateWithCUMask(&lu->stream2, (uint32_t)words, ms.data()));
-        LSX_HIP(ctx, hipExtStreamCreateWithCUMask(&lu->stream_upd, (uint32_t)words, mc.data()));
-    } else {
-        LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
+        masked = hipExtStreamCreateWithCUMask(&lu->stream2, (uint32_t)words, ms.data()) == hipSuccess &&
+                 hipExtStreamCreateWithCUMask(&lu->stream_upd, (uint32_t)words, mc.data()) == hipSuccess;
+        if (!masked) {  // no CU partition: plain streams
+            (void)hipGetLastError();
+            if (lu->stream2) (void)hipStreamDestroy(lu->stream2);
+            if (lu->stream_upd) (void)hipStreamDestroy(lu->stream_upd);
+            lu->stream2 = lu->stream_upd = nullptr;
+        }
     }
+    if (!masked) LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_start, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_next, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_panel, hipEventDisableTiming));
