@@ -198,13 +198,18 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
             pivot_bookkeeping(k, grow, pv);
             const T l = scale(v[0], pv);
             const int lim = live ? PW - 1 - c : 0;  // slots 1..lim hold columns still to eliminate
-            // in groups of 8 slots so that the pivot-row LDS reads are not all hoisted at once
+            // in groups of 8 slots: the group's pivot-row values are all read (unconditionally, so
+            // the reads overlap) before the updates, which select instead of branch
 #pragma unroll
-            for (int p0 = 0; p0 < PW; p0 += 8) {
+            for (int p0 = 0; p0 < PW - 1; p0 += 8) {
+                T pr[8];
+#pragma unroll
+                for (int p = p0; p < p0 + 8 && p < PW - 1; ++p) pr[p - p0] = prow[p + 1];
 #pragma unroll
                 for (int p = p0; p < p0 + 8 && p < PW - 1; ++p) {
                     const T nv = v[p + 1];
-                    v[p] = (p < lim) ? fma(-l, prow[p + 1], nv) : nv;
+                    const T u = fma(-l, pr[p - p0], nv);
+                    v[p] = (p < lim) ? u : nv;
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -214,13 +219,16 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
         // pending update (column c - 1): pend = this row takes it, multiplier lp, pivot row pprev
         bool pend = false;
         T lp = T(0);
-        const T* pprev = nullptr;
+        const T* pprev = reinterpret_cast<const T*>(s_rec[1] + 4);  // read, unused, before column 1
 #pragma unroll 1
         for (int c = 0; c < a.W; ++c) {
             const int par = c & 1;
             const int k = (int)(a.j0 + c);
             const int plim = PW - c;  // the pending update covers slots [0, plim)
-            if (pend) v[0] = fma(-lp, pprev[1], v[0]);
+            {
+                const T u0 = fma(-lp, pprev[1], v[0]);
+                v[0] = pend ? u0 : v[0];
+            }
             const uint64_t u = live ? pivot_key(v[0]) : 0;
             const int wl = wave_argmax(u, lr);
             if (lane == wl) {  // raw row (slot 0 already current) and its pending multiplier
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
                 // word tid of this workgroup's record, the candidate row brought up to date
                 const uint32_t* win = s_cand[par][wi];
                 uint32_t word = win[tid];
-                if (tid >= 4 && pprev) {
+                if (tid >= 4 && c > 0) {
                     const int q = (tid - 4) / WPE;
                     if (q > 0 && q < plim) {
                         const T x = fma(-(T)s_candl[par][wi], pprev[q + 1], reinterpret_cast<const T*>(win + 4)[q]);
@@ -266,35 +274,41 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // every thread reads a share of the words of all P records: all of its loads are
-            // issued at once, then the ones not yet current are re-polled
-            constexpr int MAXW = (PMAX * RW + BS - 1) / BS;
+            // every thread reads a share of the words of all P records, CW words per chunk (one
+            // chunk for P <= 32): all words of the chunk that are not yet current are re-read
+            // together, round by round (the update above leaves the records time to arrive).
+            constexpr int CW = (32 * RW + BS - 1) / BS;
             const int total = P * RW;
-            uint64_t x[MAXW];
+#pragma unroll 1
+            for (int base = 0; base < total; base += CW * BS) {
+                uint64_t x[CW];
+                uint32_t stale = 0;  // bit i: word i of this chunk still to read
 #pragma unroll
-            for (int i = 0; i < MAXW; ++i) {
-                const int idx = tid + i * BS;
-                const int rr = idx / RW;
-                x[i] = (idx < total) ? ld_relaxed_u64(g + (size_t)rr * RECS + (idx - rr * RW)) : 0;
-            }
+                for (int i = 0; i < CW; ++i)
+                    if (base + tid + i * BS < total) stale |= 1u << i;
+                const int64_t t0 = wall_clock64();
+                while (stale) {
 #pragma unroll
-            for (int i = 0; i < MAXW; ++i) {
-                const int idx = tid + i * BS;
-                if (idx >= total) continue;
-                const int rr = idx / RW;
-                const uint64_t* src = g + (size_t)rr * RECS + (idx - rr * RW);
-                if ((uint32_t)(x[i] >> 32) != seq) {
-                    const int64_t t0 = wall_clock64();
-                    while (true) {
-                        x[i] = ld_relaxed_u64(src);
-                        if ((uint32_t)(x[i] >> 32) == seq) break;
+                    for (int i = 0; i < CW; ++i) {
+                        const int idx = base + tid + i * BS;
+                        const int rr = idx / RW;
+                        if (stale & (1u << i)) x[i] = ld_relaxed_u64(g + (size_t)rr * RECS + (idx - rr * RW));
+                    }
+#pragma unroll
+                    for (int i = 0; i < CW; ++i) {
+                        if ((stale & (1u << i)) && (uint32_t)(x[i] >> 32) == seq) {
+                            s_rec[par][base + tid + i * BS] = (uint32_t)x[i];
+                            stale &= ~(1u << i);
+                        }
+                    }
+                    if (stale) {
                         if (wall_clock64() - t0 > a.timeout) {
                             atomicOr((unsigned long long*)&a.dinfo[1], 1ull);
                             break;
                         }
+                        __builtin_amdgcn_s_sleep(2);
                     }
                 }
-                s_rec[par][idx] = (uint32_t)x[i];
             }
             __syncthreads();
             // winner among the P records: lane l looks at records l and l + 64
@@ -342,63 +356,78 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
 // list of (destination row, source row) pairs: after the swaps, row dest[i] holds the former
 // row src[i].  Optionally applies the same permutation to perm[] (row i of P*A = row perm[i]).
 // One workgroup of 256 threads; kb <= 128.
+// Positions 0..kb-1 stand for rows k0..k0+kb-1, positions kb..T-1 for the distinct pivot rows
+// below the block (first occurrences, in order); swap u exchanges positions u and pos[u] >= u.
+// Position q is last written by the last swap v with pos[v] = q, which moved there what
+// position v held just before swap v, so every final label follows such links backwards in time
+// independently of the others: no sequential replay of the swaps.
 static __global__ __launch_bounds__(256) void k_perm_build(const int64_t* __restrict ipiv, int64_t k0, int kb,
                                                     int64_t* __restrict list, int* __restrict count,
                                                     int* __restrict perm) {
     __shared__ int64_t s_p[NBO];
     __shared__ int s_first[NBO];
     __shared__ int s_pos[NBO];
-    __shared__ int s_slot[NBO];
+    __shared__ int s_prev[NBO];         // last earlier swap with the same target position
+    __shared__ int s_last[2 * NBO];     // last swap targeting each position
     __shared__ int64_t s_rows[2 * NBO];
-    __shared__ int s_lab[2 * NBO];
-    __shared__ int s_T;
     __shared__ int s_pv[2 * NBO];
     const int t = threadIdx.x;
     const int64_t top = k0 + kb;
     if (t < kb) s_p[t] = ipiv[k0 + t] - 1;
+    if (t < 2 * NBO) s_last[t] = -1;
     __syncthreads();
-    int fidx = -1;
-    if (t < kb) {
+    // every loop below runs to its end (no early exit), so its LDS reads are independent
+    int fidx = t;
+    if (t < kb) {  // first swap with the same pivot row (below the block)
         const int64_t p = s_p[t];
-        if (p >= top) {
-            fidx = t;
-            for (int u = 0; u < t; ++u)
-                if (s_p[u] == p) { fidx = u; break; }
-        }
-        s_first[t] = (fidx == t) ? 1 : 0;
+        for (int u = t - 1; u >= 0; --u)
+            if (s_p[u] == p) fidx = u;
+        s_first[t] = (p >= top && fidx == t) ? 1 : 0;
+        s_rows[t] = k0 + t;
     }
     __syncthreads();
-    if (t == 0) {
-        int cnt = 0;
-        for (int u = 0; u < kb; ++u) {
-            s_slot[u] = s_first[u] ? (kb + cnt) : -1;
-            if (s_first[u]) s_rows[kb + cnt] = s_p[u];
-            cnt += s_first[u];
-        }
-        s_T = kb + cnt;
+    int extra = 0, before = 0;
+    for (int u = 0; u < kb; ++u) {
+        const int f = s_first[u];
+        extra += f;
+        before += (u < fidx) ? f : 0;
     }
-    if (t < kb) s_rows[t] = k0 + t;
-    __syncthreads();
-    if (t < kb) s_pos[t] = (fidx < 0) ? (int)(s_p[t] - k0) : s_slot[fidx];
-    const int T = s_T;
-    if (t < T) s_lab[t] = t;
-    __syncthreads();
-    if (t == 0) {
-        for (int u = 0; u < kb; ++u) {
-            const int q = s_pos[u];
-            const int x = s_lab[u];
-            s_lab[u] = s_lab[q];
-            s_lab[q] = x;
-        }
+    const int T = kb + extra;
+    if (t < kb) {  // target position of swap t
+        const int64_t p = s_p[t];
+        const int pos = p >= top ? kb + before : (int)(p - k0);
+        if (p >= top && fidx == t) s_rows[kb + before] = p;
+        s_pos[t] = pos;
+        atomicMax(&s_last[pos], t);
     }
     __syncthreads();
+    if (t < kb) {
+        const int pos = s_pos[t];
+        int prev = -1;
+        for (int u = 0; u < t; ++u)
+            if (s_pos[u] == pos) prev = u;
+        s_prev[t] = prev;
+    }
+    __syncthreads();
+    int lab = t;
     if (t < T) {
+        // label of position t: position q at time tm took, from the last swap v < tm targeting q,
+        // what position v held just before swap v
+        int q = t < kb ? s_pos[t] : t, tm = t < kb ? t : kb;
+        while (true) {
+            int v = s_last[q];
+            while (v >= tm) v = s_prev[v];
+            if (v < 0) break;
+            q = v;
+            tm = v;
+        }
+        lab = q;
         list[t] = s_rows[t];
-        list[2 * NBO + t] = s_rows[s_lab[t]];
+        list[2 * NBO + t] = s_rows[lab];
     }
     if (t == 0) *count = T;
     if (perm) {
-        if (t < T) s_pv[t] = perm[s_rows[s_lab[t]]];
+        if (t < T) s_pv[t] = perm[s_rows[lab]];
         __syncthreads();
         if (t < T) perm[s_rows[t]] = s_pv[t];
     }
@@ -458,9 +487,9 @@ __global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict A, int64_t ld,
         s_src[i] = list[2 * NBO + i];
     }
     if (trsm)
-        for (int idx = t; idx < kb * kb; idx += 256) {
-            const int kk = idx / kb, i = idx - kk * kb;
-            s_L[i][kk] = A[(k0 + kk) * ld + k0 + i];
+        for (int idx = t; idx < PW * PW; idx += 256) {
+            const int kk = idx / PW, i = idx - kk * PW;
+            s_L[i][kk] = (i < kb && kk < kb) ? A[(k0 + kk) * ld + k0 + i] : T(0);
         }
     __syncthreads();
     const int64_t cb = c0 + (int64_t)blockIdx.x * LCW;
@@ -477,11 +506,18 @@ __global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict A, int64_t ld,
         if (cc < ncol) A[(cb + cc) * ld + s_dst[i]] = s_buf[i][cc];
     }
     if (!trsm) return;
-    // forward substitution with the unit-lower kb x kb block, one thread per column
-    if (t < LCW) {
+    // forward substitution with the unit-lower kb x kb block (kb <= PW): thread (col, group)
+    // updates rows group + 8j of its column, one barrier per pivot row
+    {
+        const int col = t & (LCW - 1), grp = t >> 5;
         for (int kk = 0; kk < kb - 1; ++kk) {
-            const T xk = s_buf[kk][t];
-            for (int i = kk + 1; i < kb; ++i) s_buf[i][t] = fma(-s_L[i][kk], xk, s_buf[i][t]);
+            const T xk = s_buf[kk][col];
+#pragma unroll
+            for (int j = 0; j < PW / 8; ++j) {
+                const int i = grp + 8 * j;
+                if (i > kk && i < kb) s_buf[i][col] = fma(-s_L[i][kk], xk, s_buf[i][col]);
+            }
+            __syncthreads();
         }
     }
     __syncthreads();
@@ -495,7 +531,9 @@ __global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict A, int64_t ld,
 // Inverses of the 64 x 64 diagonal blocks of L (unit lower) and U (upper) of the factored
 // matrix, used by the blocked TRSM of the outer update and by the triangular solves.
 // grid = (number of blocks, 2): y = 0 inverts L blocks, y = 1 U blocks.  One thread per
-// column of the inverse (columns are independent), the block staged in LDS.
+// column of the inverse (columns are independent), the block staged in LDS and read by
+// broadcast; the column being built stays in registers (loops fully unrolled) and each
+// substitution sum is split over four partial sums.
 template <typename T>
 __global__ __launch_bounds__(64) void k_diag_inv(const T* __restrict F, int64_t ld, int64_t n,
                                                  int64_t b0, T* __restrict DL, T* __restrict DU) {
@@ -504,30 +542,47 @@ __global__ __launch_bounds__(64) void k_diag_inv(const T* __restrict F, int64_t 
     const int nb = (int)min((int64_t)DB, n - r0);
     const int c = threadIdx.x;
     __shared__ T D[DB][DB + 1];   // D[i][k]
-    __shared__ T X[DB][DB + 1];   // X[i][c]
+#pragma unroll 16
     for (int k = 0; k < DB; ++k) D[c][k] = (c < nb && k < nb) ? F[(r0 + k) * ld + r0 + c] : (c == k ? T(1) : T(0));
     __syncthreads();
+    T x[DB];
     if (blockIdx.y == 0) {
-        // column c of L^{-1}: x_c = 1, x_i = -sum_{k=c}^{i-1} L[i,k] x_k (i > c)
-        for (int i = 0; i < DB; ++i) X[i][c] = (i == c) ? T(1) : T(0);
-        for (int i = c + 1; i < DB; ++i) {
-            T acc = T(0);
-            for (int k = c; k < i; ++k) acc = fma(D[i][k], X[k][c], acc);
-            X[i][c] = -acc;
+        // column c of L^{-1}: x_c = 1, x_i = -sum_{k=c}^{i-1} L[i,k] x_k (i > c); x_k = 0 for k < c
+#pragma unroll
+        for (int i = 0; i < DB; ++i) {
+            T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+#pragma unroll
+            for (int k = 0; k + 3 < i; k += 4) {
+                a0 = fma(D[i][k], x[k], a0);
+                a1 = fma(D[i][k + 1], x[k + 1], a1);
+                a2 = fma(D[i][k + 2], x[k + 2], a2);
+                a3 = fma(D[i][k + 3], x[k + 3], a3);
+            }
+#pragma unroll
+            for (int k = i & ~3; k < i; ++k) a0 = fma(D[i][k], x[k], a0);
+            x[i] = (i < c) ? T(0) : (i == c ? T(1) : -((a0 + a1) + (a2 + a3)));
         }
     } else {
-        // column c of U^{-1}: x_c = 1/U[c,c], x_i = -(sum_{k=i+1}^{c} U[i,k] x_k) / U[i,i] (i < c)
-        for (int i = 0; i < DB; ++i) X[i][c] = T(0);
-        X[c][c] = T(1) / D[c][c];
-        for (int i = c - 1; i >= 0; --i) {
-            T acc = T(0);
-            for (int k = i + 1; k <= c; ++k) acc = fma(D[i][k], X[k][c], acc);
-            X[i][c] = -acc / D[i][i];
+        // column c of U^{-1}: x_c = 1/U[c,c], x_i = -(sum_{k=i+1}^{c} U[i,k] x_k) / U[i,i] (i < c);
+        // x_k = 0 for k > c
+#pragma unroll
+        for (int i = DB - 1; i >= 0; --i) {
+            T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+#pragma unroll
+            for (int k = i + 1; k + 3 < DB; k += 4) {
+                a0 = fma(D[i][k], x[k], a0);
+                a1 = fma(D[i][k + 1], x[k + 1], a1);
+                a2 = fma(D[i][k + 2], x[k + 2], a2);
+                a3 = fma(D[i][k + 3], x[k + 3], a3);
+            }
+#pragma unroll
+            for (int k = i + 1 + ((DB - 1 - i) & ~3); k < DB; ++k) a0 = fma(D[i][k], x[k], a0);
+            x[i] = (i > c) ? T(0) : (i == c ? T(1) / D[c][c] : -((a0 + a1) + (a2 + a3)) / D[i][i]);
         }
     }
-    __syncthreads();
     T* out = (blockIdx.y == 0 ? DL : DU) + blk * DB * DB;
-    for (int i = 0; i < DB; ++i) out[c * DB + i] = X[i][c];  // column-major 64 x 64
+#pragma unroll
+    for (int i = 0; i < DB; ++i) out[c * DB + i] = x[i];  // column-major 64 x 64
 }
 
 // ------------------------------------------------------------------------------------------
