@@ -317,7 +317,19 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-gmres-maxiters", type=int, default=300)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (HBM byte-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 8, "copy_bytes_written_per_launch": src.numel() * 8,
+                          "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the dense LU would need a 2D block-cyclic layout with per-column pivot all-reduces)",
@@ -389,12 +401,16 @@ def main():
                                   ms_per_step=prof.ms[i] / args.steps)
     g = _abi.PROFILE_CLASSES.index("gemm")
     gemm_tflops = prof.flops[g] / (prof.ms[g] / 1e3) / 1e12 if prof.ms[g] > 0 else 0.0
-    roofline = dict(bound="mfma", kernel="k_gemm_f64 (trailing update A22 -= L21*U12)",
+    kname = "k_gemm_f64<0, 128, 128>"
+    roofline = dict(bound="mfma", kernel=f"{kname} (trailing update A22 -= L21*U12, 128x128 tiles, K=128)",
                     achieved=round(gemm_tflops, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(gemm_tflops / FP64_PEAK_TFLOPS, 4),
                     flops_per_launch=prof.flops[g] / max(prof.launches[g], 1),
+                    algorithmic_bytes_per_launch=prof.bytes[g] / max(prof.launches[g], 1),
                     avg_launch_us=prof.ms[g] / max(prof.launches[g], 1) * 1e3,
-                    traffic=traffic_from_profiles("k_gemm_f64"))
+                    launches_per_step=prof.launches[g] / args.steps,
+                    traffic=traffic_from_profiles(kname),
+                    traffic_source="profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)")
 
     result = {
         "metric": "solves/s, TFLOP/s at <=1e-13 bwd err: mixed-IR vs FP64 LU; batched p50/p99 ms",

@@ -89,14 +89,16 @@ int lsx_lu_get_factors(lsx_lu* lu, double* F, int64_t ldf, int64_t* ipiv);
 int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
 
 /* Per-kernel-class device timing of a handle (HIP events around each launch while enabled).
- * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update, 3 permutation lists,
- * 4 triangular solves, 5 refinement residual/update, 6 low-precision conversion, 7 other.
- * flops/bytes are the algorithmic work of the launches timed. */
+ * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update (128 x 128 tiles),
+ * 3 permutation lists, 4 triangular solves, 5 refinement residual/update, 6 low-precision
+ * conversion, 7 other, 8 small MFMA products (64 x 64 tiles: narrow updates and the TRSM
+ * products).  flops/bytes are the algorithmic work of the launches timed. */
+#define LSX_PROFILE_CLASSES 9
 typedef struct {
-    int64_t launches[8];
-    double ms[8];
-    double flops[8];
-    double bytes[8];
+    int64_t launches[LSX_PROFILE_CLASSES];
+    double ms[LSX_PROFILE_CLASSES];
+    double flops[LSX_PROFILE_CLASSES];
+    double bytes[LSX_PROFILE_CLASSES];
 } lsx_profile;
 
 /* enable = 1 starts timing (and clears the counters), 0 stops; out (nullable) receives the

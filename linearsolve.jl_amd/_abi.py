@@ -36,12 +36,13 @@ class RefineStats(ctypes.Structure):
                 ("pad", ctypes.c_int32)]
 
 
+PROFILE_CLASSES = ["panel", "laswp_trsm", "gemm", "perm", "trsv", "refine", "convert", "other", "gemm_small"]
+_NPC = len(PROFILE_CLASSES)
+
+
 class Profile(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_int64 * 8), ("ms", ctypes.c_double * 8), ("flops", ctypes.c_double * 8),
-                ("bytes", ctypes.c_double * 8)]
-
-
-PROFILE_CLASSES = ["panel", "laswp_trsm", "gemm", "perm", "trsv", "refine", "convert", "other"]
+    _fields_ = [("launches", ctypes.c_int64 * _NPC), ("ms", ctypes.c_double * _NPC), ("flops", ctypes.c_double * _NPC),
+                ("bytes", ctypes.c_double * _NPC)]
 
 
 class GmresStats(ctypes.Structure):

@@ -80,7 +80,7 @@ __device__ __forceinline__ uint64_t ll_pack(uint32_t payload, uint32_t seq) {
 // Per-kernel-class device timing: HIP events around launches while enabled, read back after
 // the owning stream has been synchronized.
 enum { PK_PANEL = 0, PK_LASWP = 1, PK_GEMM = 2, PK_PERM = 3, PK_TRSV = 4, PK_REFINE = 5, PK_CONVERT = 6,
-       PK_OTHER = 7 };
+       PK_OTHER = 7, PK_GEMM_SMALL = 8 };
 struct Profiler {
     bool on = false;
     lsx_profile acc{};

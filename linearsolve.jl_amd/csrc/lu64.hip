@@ -169,10 +169,10 @@ static void launch_gemm_f64(hipStream_t st, const double* A, int64_t lda, const 
 static void gemm_f64(lsx_lu* lu, const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
                      int64_t ldc, int64_t M, int64_t N, int64_t K, int mode) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K,
-                                  sizeof(double) * (2.0 * M * N + (double)M * K + (double)K * N));
     // 128 x 128 tiles when they give every CU two workgroups, 64 x 64 otherwise
-    const bool big = cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)lu->ctx->num_cus;
+    const bool big = mode == 0 && cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)lu->ctx->num_cus;
+    const int pi = lu->prof.begin(lu->cur, big ? PK_GEMM : PK_GEMM_SMALL, 2.0 * M * N * K,
+                                  sizeof(double) * (2.0 * M * N + (double)M * K + (double)K * N));
     if (mode == 0) {
         if (big) launch_gemm_f64<0, 128, 128>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K);
         else launch_gemm_f64<0, 64, 64>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K);
