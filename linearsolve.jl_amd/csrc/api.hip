@@ -70,11 +70,11 @@ int lu64_alloc(lsx_lu* lu) {
     // working type fits in LDS; FP32 is the smaller type)
     if ((n - NBO) * (int64_t)sizeof(float) <= LGMAX_BYTES && n > NBO)
         LSX_HIP(ctx, hipMalloc(&lu->pinv, sizeof(int) * cdiv(n, NBO) * n));
-    LSX_HIP(ctx, hipMalloc(&lu->tflags, sizeof(uint32_t) * 2 * nblk));
+    LSX_HIP(ctx, hipMalloc(&lu->tmsg, sizeof(uint64_t) * nblk * 2 * DB * TRHS));
     LSX_HIP(ctx, hipMalloc(&lu->DL, sizeof(double) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&lu->DU, sizeof(double) * DB * DB * nblk));
     LSX_HIP(ctx, hipMemsetAsync(lu->slots, 0, sizeof(uint64_t) * 2 * (PMAX + 1) * RECS, lu->stream));
-    LSX_HIP(ctx, hipMemsetAsync(lu->tflags, 0, sizeof(uint32_t) * 2 * nblk, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->tmsg, 0, sizeof(uint64_t) * nblk * 2 * DB * TRHS, lu->stream));
     LSX_HIP(ctx, hipMemsetAsync(lu->dinfo, 0, sizeof(int64_t) * 2, lu->stream));
     LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
     return LSX_OK;
@@ -91,7 +91,7 @@ void lu64_free(lsx_lu* lu) {
     (void)hipFree(lu->plist);
     (void)hipFree(lu->pcount);
     (void)hipFree(lu->pinv);
-    (void)hipFree(lu->tflags);
+    (void)hipFree(lu->tmsg);
     (void)hipFree(lu->DL);
     (void)hipFree(lu->DU);
     (void)hipFree(lu->wb);

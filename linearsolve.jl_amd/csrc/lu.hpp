@@ -38,7 +38,7 @@ struct lsx_lu {
     int* pcount = nullptr;       // [0] inner T, [1], [2] outer T by block parity
     int* pinv = nullptr;         // per outer block: inverse of the row map after that block
                                  // (the deferred row interchanges of the L columns)
-    uint32_t* tflags = nullptr;  // triangular-solve block flags (2 * nblk)
+    uint64_t* tmsg = nullptr;    // triangular-solve block messages (nblk * TMW words)
     double* DL = nullptr;        // inverses of the 64 x 64 diagonal blocks of L (nblk x 64 x 64)
     double* DU = nullptr;        // ... and of U
     uint32_t seq = 1;

@@ -174,7 +174,7 @@ int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, 
     const int64_t n = lu->n;
     const int nblk = (int)cdiv(n, DB);
     if (lu->tseq > 0xF0000000u) {
-        LSX_HIP(ctx, hipMemsetAsync(lu->tflags, 0, sizeof(uint32_t) * 2 * nblk, lu->stream));
+        LSX_HIP(ctx, hipMemsetAsync(lu->tmsg, 0, sizeof(uint64_t) * nblk * TMW, lu->stream));
         lu->tseq = 1;
     }
     LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
@@ -184,9 +184,9 @@ int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, 
         const uint32_t s1 = lu->tseq++, s2 = lu->tseq++;
         const int pi = lu->prof.begin(lu->stream, PK_TRSV, 2.0 * n * n * nr, sizeof(T) * (double)n * n);
         hipLaunchKernelGGL((k_trsv<T, true>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DL,
-                           dB + c0 * ldb, ldb, lu->wy, n, nr, lu->tflags, s1, timeout, lu->dinfo + 1);
+                           dB + c0 * ldb, ldb, lu->wy, n, nr, lu->tmsg, s1, timeout, lu->dinfo + 1);
         hipLaunchKernelGGL((k_trsv<T, false>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DU,
-                           (const double*)lu->wy, n, dX + c0 * ldx, ldx, nr, lu->tflags + nblk, s2, timeout,
+                           (const double*)lu->wy, n, dX + c0 * ldx, ldx, nr, lu->tmsg, s2, timeout,
                            lu->dinfo + 1);
         lu->prof.end(lu->stream, pi);
     }

@@ -586,24 +586,30 @@ __global__ __launch_bounds__(64) void k_diag_inv(const T* __restrict F, int64_t 
 }
 
 // ------------------------------------------------------------------------------------------
-// Triangular solves (getrs), one workgroup per DB-row block, blocks chained by flags.
+// Triangular solves (getrs), one workgroup per DB-row block.
 //   LOWER: Out = L^{-1} (P In)  (unit lower; the permutation gathers In by perm)
 //   UPPER: Out = U^{-1} In
-// Each block accumulates the contributions of the already solved blocks as they are published
-// (four k-quarters of partial sums per row), then applies the inverse of its diagonal block.
+// Each block publishes its solution (DB x nrhs FP64 values) as flag-in-word messages: two
+// 64-bit words per value, its 32-bit halves each with the launch's sequence number.  The blocks
+// after it read those words directly, so a dependency costs one memory round trip (no flags, no
+// fences).  A block accumulates the contributions of the blocks before it in solve order (four
+// k-quarters of partial sums per row), the message reads of the next block issued before the
+// current one is used, then applies the inverse of its diagonal block.
+constexpr int TMW = 2 * DB * TRHS;  // message words of one block
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld, int64_t n,
                                               const int* __restrict perm, const T* __restrict Dinv,
                                               const double* In, int64_t ldin, double* Out, int64_t ldout,
-                                              int nrhs, uint32_t* flags, uint32_t seq, int64_t timeout,
+                                              int nrhs, uint64_t* msg, uint32_t seq, int64_t timeout,
                                               int64_t* err) {
+    constexpr int WPT = TMW / 256;  // message words per thread
     const int nblk = (int)((n + DB - 1) / DB);
     const int bw = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
     const int64_t r0 = (int64_t)bw * DB;
     const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
     const int64_t row = r0 + rr;
     __shared__ double Ds[DB][DB + 1];
-    __shared__ double Ys[DB][TRHS + 1];
+    __shared__ double Ys[2][DB][TRHS + 1];
     __shared__ double Ps[4][DB][TRHS + 1];
     for (int idx = tid; idx < DB * DB; idx += 256) {
         const int i = idx & (DB - 1), k = idx >> 6;
@@ -612,56 +618,93 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
     double acc[TRHS];
 #pragma unroll
     for (int s = 0; s < TRHS; ++s) acc[s] = 0.0;
-    const int jbeg = LOWER ? 0 : nblk - 1, jend = LOWER ? bw : bw, jstep = LOWER ? 1 : -1;
-    for (int j = jbeg; j != jend; j += jstep) {
+    // word w of a block's message: value (row k, rhs s) half h at w = 2 (s DB + k) + h
+    const int nj = LOWER ? bw : nblk - 1 - bw;  // blocks before this one in solve order
+    auto jblk = [&](int t) { return LOWER ? t : nblk - 1 - t; };
+    uint64_t x[WPT];
+    auto issue = [&](int t) {
+        const uint64_t* m = msg + (size_t)jblk(t) * TMW;
+#pragma unroll
+        for (int i = 0; i < WPT; ++i) {
+            const int w = tid + 256 * i;
+            x[i] = (w / (2 * DB) < nrhs) ? ld_relaxed_u64(m + w) : 0;
+        }
+    };
+    if (nj > 0) issue(0);
+    for (int t = 0; t < nj; ++t) {
+        const int j = jblk(t), buf = t & 1;
         double lv[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int64_t col = (int64_t)j * DB + 16 * g + q;
             lv[q] = (row < n && col < n) ? (double)F[col * ld + row] : 0.0;
         }
-        if (tid == 0 && ld_relaxed_u32(&flags[j]) != seq) {
+        // words of block j not yet current are read again, all together, round by round
+        const uint64_t* m = msg + (size_t)j * TMW;
+        uint32_t stale = 0;
+#pragma unroll
+        for (int i = 0; i < WPT; ++i)
+            if ((tid + 256 * i) / (2 * DB) < nrhs && (uint32_t)(x[i] >> 32) != seq) stale |= 1u << i;
+        if (stale) {
             const int64_t t0 = wall_clock64();
-            while (ld_relaxed_u32(&flags[j]) != seq) {
-                if (wall_clock64() - t0 > timeout) { atomicOr((unsigned long long*)err, 2ull); break; }
+            while (stale) {
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int i = 0; i < WPT; ++i)
+                    if (stale & (1u << i)) x[i] = ld_relaxed_u64(m + tid + 256 * i);
+#pragma unroll
+                for (int i = 0; i < WPT; ++i)
+                    if ((stale & (1u << i)) && (uint32_t)(x[i] >> 32) == seq) stale &= ~(1u << i);
+                if (stale && wall_clock64() - t0 > timeout) {
+                    atomicOr((unsigned long long*)err, 2ull);
+                    break;
+                }
             }
         }
-        __syncthreads();
-        for (int idx = tid; idx < DB * TRHS; idx += 256) {
-            const int kk = idx & (DB - 1), s = idx >> 6;
-            const int64_t gr = (int64_t)j * DB + kk;
-            Ys[kk][s] = (s < nrhs && gr < n) ? ld_coherent_f64(Out + (int64_t)s * ldout + gr) : 0.0;
+        // the two halves of a value sit in words w, w + 1 of threads tid, tid ^ 1
+#pragma unroll
+        for (int i = 0; i < WPT; ++i) {
+            const int w = tid + 256 * i;
+            const uint32_t mine = (uint32_t)x[i];
+            const uint32_t other = (uint32_t)__shfl_xor((int)mine, 1);
+            const int k = (w >> 1) & (DB - 1), s = w / (2 * DB);
+            if (!(w & 1) && s < nrhs) Ys[buf][k][s] = __hiloint2double((int)other, (int)mine);
         }
+        if (t + 1 < nj) issue(t + 1);  // the next block's words travel while this one is used
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 16; ++q)
 #pragma unroll
-            for (int s = 0; s < TRHS; ++s) acc[s] = fma(lv[q], Ys[16 * g + q][s], acc[s]);
-        __syncthreads();
+            for (int s = 0; s < TRHS; ++s) acc[s] = fma(lv[q], Ys[buf][16 * g + q][s], acc[s]);
+        // Ys is double-buffered: buffer buf is written again two iterations on, after the next
+        // barrier
     }
 #pragma unroll
     for (int s = 0; s < TRHS; ++s) Ps[g][rr][s] = acc[s];
     __syncthreads();
     // right-hand side minus contributions, one (row, rhs) per thread-slot
+    double (*Yb)[TRHS + 1] = Ys[nj & 1];
     for (int idx = tid; idx < DB * TRHS; idx += 256) {
         const int i = idx & (DB - 1), s = idx >> 6;
         const int64_t gr = r0 + i;
         double b = 0.0;
         if (gr < n && s < nrhs) b = LOWER ? In[(int64_t)s * ldin + perm[gr]] : In[(int64_t)s * ldin + gr];
-        Ys[i][s] = b - (Ps[0][i][s] + Ps[1][i][s] + Ps[2][i][s] + Ps[3][i][s]);
+        Yb[i][s] = b - (Ps[0][i][s] + Ps[1][i][s] + Ps[2][i][s] + Ps[3][i][s]);
     }
     __syncthreads();
+    uint64_t* mine = msg + (size_t)bw * TMW;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int s = g + 4 * q;
         double y = 0.0;
 #pragma unroll 8
-        for (int k = 0; k < DB; ++k) y = fma(Ds[rr][k], Ys[k][s], y);
-        if (row < n && s < nrhs) Out[(int64_t)s * ldout + row] = y;
+        for (int k = 0; k < DB; ++k) y = fma(Ds[rr][k], Yb[k][s], y);
+        if (s < nrhs) {
+            if (row < n) Out[(int64_t)s * ldout + row] = y;
+            st_relaxed_u64(mine + 2 * (s * DB + rr), ll_pack((uint32_t)__double2loint(y), seq));
+            st_relaxed_u64(mine + 2 * (s * DB + rr) + 1, ll_pack((uint32_t)__double2hiint(y), seq));
+        }
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[bw], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace lsx

@@ -1,5 +1,5 @@
 // lu_mixed.hip — mixed-precision LU: factor in FP32 (FP32 MFMA trailing update) or with BF16
-// trailing updates (BF16 operands with a per-panel power-of-two scale, FP32 accumulation), then
+// trailing updates (BF16 operands with per-column power-of-two scales, FP32 accumulation), then
 // recover FP64 accuracy by iterative refinement (IR) or GMRES-IR with FP64 residuals.
 //
 // The API shape follows OpenBLAS32MixedLUFactorization (src/openblas.jl:461-543: cast A to
@@ -12,6 +12,7 @@
 #include <cmath>
 #include <vector>
 
+#include "gemm.hpp"
 #include "lu_driver.hpp"
 
 namespace lsx {
@@ -23,7 +24,8 @@ struct MixedState {
     float* DL = nullptr;     // FP32 inverses of diagonal blocks
     float* DU = nullptr;
     double* anorm = nullptr; // device: ||A||_inf
-    float* scale = nullptr;  // device: per-update power-of-two scale of the U12 operand (BF16 path)
+    float* scale = nullptr;  // device: per-column power-of-two scales of the U12 operand (BF16 path),
+                             // n floats per getrf stream
     double* r = nullptr;     // n x cap residuals / corrections
     double* xw = nullptr;    // n x cap iterate
     double* bw = nullptr;    // n x cap right-hand sides
@@ -47,112 +49,21 @@ __global__ void k_cast_f32(const double* __restrict src, int64_t lds, float* __r
         dst[col * ldd + i] = (float)src[col * lds + i];
 }
 
-// Power-of-two scale 2^-e with 2^e >= max |U| over a K x N block (column-major, ld), so the
-// scaled operand has magnitude <= 1 before rounding to BF16.
-__global__ void k_absmax_scale(const float* __restrict U, int64_t ld, int K, int N, float* __restrict out) {
-    __shared__ float red[256];
+// Per-column power-of-two scales of the U12 block of an update (K x N, column-major, ld):
+// out[j] = 2^-e with 2^e >= max_k |U[k, j]|, so every scaled column has magnitude <= 1 before it
+// is rounded to BF16 (and the scaling is exact).  One thread per column.
+__global__ void k_colmax_scale(const float* __restrict U, int64_t ld, int K, int N, float* __restrict out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const float* col = U + (int64_t)j * ld;
     float m = 0.f;
-    for (int64_t idx = threadIdx.x; idx < (int64_t)K * N; idx += 256) {
-        const int64_t c = idx / K, k = idx - c * K;
-        m = fmaxf(m, fabsf(U[c * ld + k]));
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const float mx = red[0];
-        int e = 0;
-        if (mx > 0.f && isfinite(mx)) (void)frexpf(mx, &e);
-        *out = ldexpf(1.f, -e);
-    }
+    for (int k = 0; k < K; ++k) m = fmaxf(m, fabsf(col[k]));
+    int e = 0;
+    if (m > 0.f && isfinite(m)) (void)frexpf(m, &e);
+    out[j] = ldexpf(1.f, -e);
 }
 
 constexpr int MBM = 128, MBN = 128;
-
-// C -= A B (mode 0) or C = A B (mode 1, in place on B when M <= 128), FP32 MFMA 16x16x4.
-// Same tiling as the FP64 kernel; lane l / slot q holds C(rb + (l & 15), cb + 4 (l >> 4) + q).
-__global__ __launch_bounds__(256) void k_gemm_f32(const float* __restrict Ag, int64_t lda, const float* Bg,
-                                                  int64_t ldb, float* Cg, int64_t ldc, int M, int N, int K,
-                                                  int mode) {
-    constexpr int BK = 16;
-    const int TM = (M + MBM - 1) / MBM, TN = (N + MBN - 1) / MBN, T = TM * TN;
-    const int per = (T + 7) / 8;
-    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (tile >= T) return;
-    const int row0 = (tile % TM) * MBM, col0 = (tile / TM) * MBN;
-    __shared__ float As[2][BK][MBM + 4];
-    __shared__ float Bs[2][BK][MBN + 4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    v4f acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    float ra[8], rb[8];
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int idx = it * 256 + tid;
-            const int r = idx & (MBM - 1), k = idx >> 7;
-            ra[it] = (row0 + r < M && k0 + k < K) ? Ag[(int64_t)(k0 + k) * lda + row0 + r] : 0.f;
-            const int kb = idx & (BK - 1), c = idx >> 4;
-            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] : 0.f;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int it = 0; it < 8; ++it) {
-            const int idx = it * 256 + tid;
-            As[buf][idx >> 7][idx & (MBM - 1)] = ra[it];
-            Bs[buf][idx & (BK - 1)][idx >> 4] = rb[it];
-        }
-    };
-    const int nk = (K + BK - 1) / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int s = 0; s < nk; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < nk) gload((s + 1) * BK);
-#pragma unroll
-        for (int k4 = 0; k4 < BK / 4; ++k4) {
-            const int kr = k4 * 4 + (lane >> 4);
-            float x[4], y[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = Bs[buf][kr][wn * 64 + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) y[j] = As[buf][kr][wm * 64 + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i], y[j], acc[i][j], 0, 0, 0);
-        }
-        if (s + 1 < nk) {
-            sstore(buf ^ 1);
-            __syncthreads();
-        }
-    }
-    if (mode == 1) __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
-                if (row < M && col < N) {
-                    float* cp = Cg + (int64_t)col * ldc + row;
-                    *cp = (mode == 0) ? (*cp - acc[i][j][q]) : acc[i][j][q];
-                }
-            }
-        }
-}
 
 __device__ __forceinline__ __bf16 to_bf16(float f) {
     // round to nearest even (finite inputs)
@@ -162,10 +73,11 @@ __device__ __forceinline__ __bf16 to_bf16(float f) {
     return __builtin_bit_cast(__bf16, h);
 }
 
-// C -= A B with BF16 operands and FP32 accumulation (v_mfma_f32_16x16x32_bf16).  B (the U12
-// block) is multiplied by the power-of-two *scale before rounding and the product by 1/scale
-// after accumulation (both exact).  A (L21) holds multipliers of magnitude <= 1.
-__global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict Ag, int64_t lda, const float* Bg,
+// C -= A B with BF16 operands and FP32 accumulation (v_mfma_f32_16x16x32_bf16).  Column j of B
+// (the U12 block) is multiplied by its power-of-two scale[j] before rounding and column j of the
+// product by 1/scale[j] after accumulation (both exact).  A (L21) holds multipliers of magnitude
+// <= 1.
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16(const float* __restrict Ag, int64_t lda, const float* Bg,
                                                    int64_t ldb, float* Cg, int64_t ldc, int M, int N, int K,
                                                    const float* __restrict scale) {
     constexpr int BK = 32, PAD = 8;
@@ -178,7 +90,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict Ag, i
     __shared__ __align__(16) __bf16 Bs[2][MBN][BK + PAD];   // Bs[col][k]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
-    const float s = *scale, inv = 1.f / s;
     v4f acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -192,7 +103,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict Ag, i
             const int r = idx & (MBM - 1), k = idx >> 7;          // A: rows fastest (coalesced)
             ra[it] = (row0 + r < M && k0 + k < K) ? Ag[(int64_t)(k0 + k) * lda + row0 + r] : 0.f;
             const int kb = idx & (BK - 1), c = idx >> 5;          // B: k fastest (coalesced)
-            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] * s : 0.f;
+            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] * scale[col0 + c] : 0.f;
         }
     };
     auto sstore = [&](int buf) {
@@ -225,20 +136,34 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const float* __restrict Ag, i
             __syncthreads();
         }
     }
+    // sixteen C values at a time are loaded before any of them is stored
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        float cv[4][4], inv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
+            inv[q] = col < N ? 1.f / scale[col] : 0.f;
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int row = row0 + wm * 64 + j * 16 + (lane & 15);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
-                if (row < M && col < N) {
-                    float* cp = Cg + (int64_t)col * ldc + row;
-                    *cp = *cp - acc[i][j][q] * inv;
-                }
+                cv[j][q] = (row < M && col < N) ? Cg[(int64_t)col * ldc + row] : 0.f;
             }
         }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
+                if (row < M && col < N) Cg[(int64_t)col * ldc + row] = cv[j][q] - acc[i][j][q] * inv[q];
+            }
+        }
+    }
 }
 
 // mode 0: R = B - A X (FP64, nrhs <= TRHS columns) and, per column, partial max |r| and max |x|
@@ -375,7 +300,7 @@ int mixed_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&m->DL, sizeof(float) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&m->DU, sizeof(float) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&m->anorm, sizeof(double)));
-    LSX_HIP(ctx, hipMalloc(&m->scale, sizeof(float) * 8));
+    LSX_HIP(ctx, hipMalloc(&m->scale, sizeof(float) * 2 * (lu->n + 64)));
     const int64_t nb = cdiv(n, RRB);
     LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64)));
     LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64),
@@ -408,18 +333,17 @@ void mixed_free(lsx_lu* lu) {
 static void gemm_f32(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                      int64_t M, int64_t N, int64_t K, int mode) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
-    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
-    hipLaunchKernelGGL(k_gemm_f32, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->cur, A, lda, B, ldb, C, ldc,
-                       (int)M, (int)N, (int)K, mode);
+    const bool big = mode == 0 && cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)lu->ctx->num_cus;
+    const int pi = lu->prof.begin(lu->cur, big ? PK_GEMM : PK_GEMM_SMALL, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    gemm_dispatch<float>(lu->cur, lu->ctx->num_cus, A, lda, B, ldb, C, ldc, M, N, K, mode);
     lu->prof.end(lu->cur, pi);
 }
 
 static void gemm_bf16(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                       int64_t M, int64_t N, int64_t K) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    float* sc = ms(lu)->scale + (lu->cur == lu->stream2 ? 1 : 0);  // one per getrf stream
-    hipLaunchKernelGGL(k_absmax_scale, dim3(1), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
+    float* sc = ms(lu)->scale + (lu->cur == lu->stream2 ? lu->n + 64 : 0);  // one set per getrf stream
+    hipLaunchKernelGGL(k_colmax_scale, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
     const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
     const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
     hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->cur, A, lda, B, ldb, C,
