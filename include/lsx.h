@@ -209,10 +209,14 @@ typedef struct {
     double t_submit, t_start, t_done; /* seconds on the serving clock */
 } lsx_result;
 
-/* A continuous-batching request loop: requests are admitted while their estimated HBM footprint
- * fits hbm_budget bytes, small dense systems are grouped into batched launches, large ones and
- * sparse solves run on their own streams.  The request's host buffers must stay valid until the
- * result is polled. */
+/* A continuous-batching request loop over one GPU.  hbm_budget bytes are allocated once: a quarter
+ * (at most 2 GiB) is the arena of the batched launches, the rest a heap from which each admitted
+ * request's device memory is carved (admission = the carve succeeds, and the request's
+ * cooperating workgroups fit beside the running ones).  Dense systems with n <= 512 are grouped
+ * into batched launches, larger ones run on reusable LU handles, SimpleGMRES requests are grouped
+ * into multi-solve launches.  nstreams (2..8) lanes serve dense and sparse requests, split evenly;
+ * the batched lane comes on top.  The request's host buffers must stay valid until the result is
+ * polled.  No counterpart in the reference (LinearSolve.jl solves one problem per call). */
 int lsx_serve_create(lsx_ctx* ctx, size_t hbm_budget, int32_t nstreams, lsx_serve** out);
 int lsx_serve_destroy(lsx_serve* s);
 int lsx_serve_submit(lsx_serve* s, const lsx_request* req, double* x_out, uint64_t* ticket);

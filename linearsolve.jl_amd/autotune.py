@@ -5,7 +5,7 @@ workload's distribution, generated on the GPU), times the persistent batched lau
 candidate panel width, keeps the fastest, and writes the table to tile_table.json beside this
 file; _BatchHandle loads that file when it exists (lsx_batch_set_tiles).
 
-    python -m linearsolve_jl_amd.autotune           # or: python linearsolve.jl_amd/autotune.py
+    python linearsolve.jl_amd/autotune.py [out.json]   # default: tile_table.json beside this file
 """
 import json
 import os
@@ -93,4 +93,4 @@ def tune(bins=DEFAULT_BINS, widths=WIDTHS, per_bin=1024, reps=3, seed=0, path=No
 
 
 if __name__ == "__main__":
-    tune()
+    tune(path=sys.argv[1] if len(sys.argv) > 1 else None)

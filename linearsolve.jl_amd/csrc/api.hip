@@ -20,8 +20,9 @@ int fail(lsx_ctx* ctx, int code, const char* fmt, ...) {
 
 int lu64_alloc(lsx_lu* lu) {
     lsx_ctx* ctx = lu->ctx;
-    const int64_t n = lu->n;
-    lu->ld = pad_ld(n);
+    if (lu->cap < lu->n) lu->cap = lu->n;
+    const int64_t n = lu->cap;  // every buffer holds the largest order
+    lu->ld = pad_ld(lu->n);
     const int nblk = (int)cdiv(n, DB);
     LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream, hipStreamNonBlocking));
     // getrf lookahead streams.  The panel workgroups wait on each other, so they get CUs of their
@@ -32,7 +33,7 @@ int lu64_alloc(lsx_lu* lu) {
     // hardware queue of its own, of which a process has few)
     const int per = ctx->num_cus / 8;
     bool masked = false;
-    if (n >= 2048 && ctx->num_cus % 8 == 0 && per % 8 == 0 && per >= 32 && cdiv(n, 512) <= 32) {
+    if (lu->lookahead && n >= 2048 && ctx->num_cus % 8 == 0 && per % 8 == 0 && per >= 32 && cdiv(n, 512) <= 32) {
         const int words = (ctx->num_cus + 31) / 32;
         std::vector<uint32_t> ms(words, 0u), mc(words, 0u);
         for (int x = 0; x < 8; ++x)
@@ -46,17 +47,18 @@ int lu64_alloc(lsx_lu* lu) {
                  hipExtStreamCreateWithCUMask(&lu->stream_upd, (uint32_t)words, mc.data()) == hipSuccess;
         if (!masked) {  // no CU partition: plain streams
             (void)hipGetLastError();
-            if (lu->stream2) (void)hipStreamDestroy(lu->stream2);
+            if (lu->stream2 && lu->stream2 != lu->stream) (void)hipStreamDestroy(lu->stream2);
             if (lu->stream_upd) (void)hipStreamDestroy(lu->stream_upd);
             lu->stream2 = lu->stream_upd = nullptr;
         }
     }
-    if (!masked) LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
+    if (!lu->lookahead) lu->stream2 = lu->stream;
+    else if (!masked) LSX_HIP(ctx, hipStreamCreateWithFlags(&lu->stream2, hipStreamNonBlocking));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_start, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_next, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_panel, hipEventDisableTiming));
     lu->cur = lu->stream;
-    if (hipMalloc(&lu->F, sizeof(double) * lu->ld * n) != hipSuccess)
+    if (lu->own_F && hipMalloc(&lu->F, sizeof(double) * pad_ld(n) * n) != hipSuccess)
         return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate %lld x %lld FP64 factors", (long long)n,
                     (long long)n);
     LSX_HIP(ctx, hipMalloc(&lu->ipiv, sizeof(int64_t) * n));
@@ -80,9 +82,18 @@ int lu64_alloc(lsx_lu* lu) {
     return LSX_OK;
 }
 
+int lu64_reshape(lsx_lu* lu, int64_t n) {
+    if (lu->dtype != LSX_F64 || n < 1 || n > lu->cap)
+        return fail(lu->ctx, LSX_EINVAL, "lu reshape: order %lld outside [1, %lld]", (long long)n, (long long)lu->cap);
+    lu->n = n;
+    lu->ld = pad_ld(n);
+    lu->factored = false;
+    return LSX_OK;
+}
+
 void lu64_free(lsx_lu* lu) {
     if (lu->stream) (void)hipStreamSynchronize(lu->stream);
-    (void)hipFree(lu->F);
+    if (lu->own_F) (void)hipFree(lu->F);
     (void)hipFree(lu->ipiv);
     (void)hipFree(lu->perm);
     (void)hipFree(lu->dinfo);
@@ -101,7 +112,7 @@ void lu64_free(lsx_lu* lu) {
     if (lu->ev_start) (void)hipEventDestroy(lu->ev_start);
     if (lu->ev_next) (void)hipEventDestroy(lu->ev_next);
     if (lu->ev_panel) (void)hipEventDestroy(lu->ev_panel);
-    if (lu->stream2) (void)hipStreamDestroy(lu->stream2);
+    if (lu->stream2 && lu->stream2 != lu->stream) (void)hipStreamDestroy(lu->stream2);
     if (lu->stream_upd) (void)hipStreamDestroy(lu->stream_upd);
     if (lu->stream) (void)hipStreamDestroy(lu->stream);
 }

@@ -31,29 +31,6 @@ namespace lsx {
 constexpr int GBS = 512;            // threads per workgroup
 constexpr int GNW = GBS / 64;
 
-struct GmresArgs {
-    int64_t N;
-    const int32_t* rowptr;
-    const int32_t* col;
-    const double* val;
-    const double* b;
-    double* x;
-    const double* pl;   // nullable: Jacobi / diagonal left preconditioner
-    const double* pr;   // nullable: diagonal right preconditioner
-    double* V;          // (cap + 1) x N basis
-    double* xr;         // N: correction of the current pass (restart)
-    int memory;
-    int restart;
-    int cap;            // size of the LDS bookkeeping (memory, or maxiters when not restarted)
-    int64_t maxiters;
-    double abstol, reltol;
-    int P;
-    uint64_t* red;      // LL all-reduce slots: 2 parities x P x 2 words
-    uint32_t* bar;      // barrier counter
-    GmresOut* out;
-    int64_t timeout;
-};
-
 struct GShared {
     double wsum[2][GNW];
     uint32_t rbuf[2][2 * 256];  // 2 words per workgroup, up to 256 workgroups
@@ -73,11 +50,11 @@ struct GmresCtx {
     int64_t row0;     // first row of this workgroup
     uint32_t nred = 0, nbar = 0;
 
-    __device__ GmresCtx(const GmresArgs& a_, GShared& sh_) : a(a_), sh(sh_) {
+    __device__ GmresCtx(const GmresArgs& a_, GShared& sh_, int w_) : a(a_), sh(sh_) {
         tid = threadIdx.x;
         lane = tid & 63;
         wave = tid >> 6;
-        w = blockIdx.x;
+        w = w_;
         row0 = (int64_t)w * GBS * RPT;
     }
     __device__ int64_t row(int i) const { return row0 + tid + (int64_t)i * GBS; }
@@ -185,10 +162,9 @@ __device__ void sym_givens(double a, double b, double& c, double& s, double& rho
     }
 }
 
+// The solve on workgroup w of a.P (sh, dyn: the workgroup's static and dynamic LDS).
 template <int RPT>
-__global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
-    __shared__ GShared sh;
-    extern __shared__ double dyn[];
+__device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& sh, double* dyn) {
     const int cap = a.cap;
     double* R = dyn;                          // cap (cap + 1) / 2
     double* cs = R + (cap * (cap + 1)) / 2;   // cap
@@ -199,7 +175,7 @@ __global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
     double* qs = z + cap + 1;                 // RPT x GBS
     double* vs = qs + RPT * GBS;              // RPT x GBS
     __shared__ int s_flags[4];
-    GmresCtx<RPT> g(a, sh);
+    GmresCtx<RPT> g(a, sh, w);
     const int64_t N = a.N;
     const int tid = g.tid;
     double* x = a.x;
@@ -395,6 +371,59 @@ __global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
     }
 }
 
+template <int RPT>
+__global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
+    __shared__ GShared sh;
+    extern __shared__ double dyn[];
+    gmres_body<RPT>(a, (int)blockIdx.x, sh, dyn);
+}
+
+// Several solves in one launch (gmres_launch_multi), on a pool of workgroups that may be smaller
+// than the solves' total: a workgroup takes the next number from a counter, runs that rank of
+// its solve, and takes another number when the solve ends.  Numbers go out in start order, so
+// every solve but the one holding the highest number handed out so far has all its ranks
+// running and finishes by itself; with at least as many workgroups as the widest solve, that
+// last one fills up as the others end.
+template <int RPT>
+__global__ __launch_bounds__(GBS) void k_gmres_multi(const GmresArgs* __restrict items, const int* __restrict first,
+                                                     int nitems, uint32_t* ticket) {
+    __shared__ GShared sh;
+    __shared__ int s_id;
+    extern __shared__ double dyn[];
+    const int total = first[nitems];
+    while (true) {
+        __syncthreads();  // the previous solve's LDS is no longer read
+        if (threadIdx.x == 0) s_id = (int)atomicAdd(ticket, 1u);
+        __syncthreads();
+        const int id = __builtin_amdgcn_readfirstlane(s_id);
+        if (id >= total) return;
+        int lo = 0, hi = nitems - 1;  // last solve with first[i] <= id
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (first[mid] <= id) lo = mid;
+            else hi = mid - 1;
+        }
+        const int item = __builtin_amdgcn_readfirstlane(lo);
+        const GmresArgs a = items[item];
+        gmres_body<RPT>(a, id - first[item], sh, dyn);
+    }
+}
+
+// Zeroes the exchange area and result of every solve of a multi launch, and its ticket counter.
+__global__ void k_gmres_clear(const GmresArgs* __restrict items, uint32_t* ticket) {
+    const GmresArgs& a = items[blockIdx.x];
+    const int64_t words = gmres_red_words(a.P);
+    for (int64_t i = threadIdx.x; i < words; i += blockDim.x) a.red[i] = 0ull;
+    if (threadIdx.x == 0) {
+        a.out->retcode = 0;
+        a.out->pad = 0;
+        a.out->iters = 0;
+        a.out->resid = 0.0;
+        a.out->err = 0ull;
+        if (blockIdx.x == 0) *ticket = 0u;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 __global__ void k_spmv(int64_t N, const int32_t* __restrict rowptr, const int32_t* __restrict col,
                        const double* __restrict val, const double* __restrict xv, double* __restrict y) {
@@ -421,10 +450,7 @@ int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? LSX_OK : LSX_EHIP;
 }
 
-int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx, const double* dpl,
-                 const double* dpr, hipStream_t st) {
-    lsx_ctx* ctx = A->ctx;
-    const int64_t N = A->N;
+int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out) {
     int P = 1;
     const int rpt = pick_rpt(N, P);
     if (P > PMAX_G) return fail(ctx, LSX_EINVAL, "gmres: N = %lld exceeds %d rows", (long long)N, PMAX_G * 16 * GBS);
@@ -433,6 +459,39 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     if (cap > GMRES_MAX_CAP)
         return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld exceeds %d (restart = %d)", (long long)cap,
                     GMRES_MAX_CAP, p.restart);
+    const size_t dynlds = sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
+    if (dynlds > 150 * 1024)
+        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld too large for N = %lld", (long long)cap, (long long)N);
+    out->P = P;
+    out->rpt = rpt;
+    out->cap = (int)cap;
+    out->dynlds = dynlds;
+    return LSX_OK;
+}
+
+int gmres_launch_multi(lsx_ctx* ctx, hipStream_t st, const GmresArgs* items, const int* first, int nitems,
+                       int pool, int rpt, size_t dynlds, uint32_t* ticket) {
+    if (nitems < 1 || pool < 1) return fail(ctx, LSX_EINVAL, "gmres: empty multi launch");
+    hipLaunchKernelGGL(k_gmres_clear, dim3((unsigned)nitems), dim3(256), 0, st, items, ticket);
+    switch (rpt) {
+        case 1: hipLaunchKernelGGL(k_gmres_multi<1>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
+        case 4: hipLaunchKernelGGL(k_gmres_multi<4>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
+        default: hipLaunchKernelGGL(k_gmres_multi<16>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
+    }
+    LSX_HIP(ctx, hipGetLastError());
+    return LSX_OK;
+}
+
+int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx, const double* dpl,
+                 const double* dpr, hipStream_t st) {
+    lsx_ctx* ctx = A->ctx;
+    const int64_t N = A->N;
+    GmresPlan plan;
+    int rc = gmres_plan(ctx, N, p, &plan);
+    if (rc) return rc;
+    const int P = plan.P, rpt = plan.rpt;
+    const int64_t memory = std::max<int64_t>(1, std::min<int64_t>(p.memory, p.maxiters));
+    const int64_t cap = plan.cap;
     // workspace: basis (cap + 1) x N, correction N
     const size_t need = sizeof(double) * (size_t)N * (size_t)(cap + 2);
     if (need > A->ws_bytes) {
@@ -467,9 +526,7 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     g.bar = reinterpret_cast<uint32_t*>(A->red + 4 * PMAX_G);
     g.out = A->out;
     g.timeout = ctx->wall_khz * 20000;  // 20 s
-    const size_t dynlds = sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
-    if (dynlds > 150 * 1024)
-        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld too large for N = %lld", (long long)cap, (long long)N);
+    const size_t dynlds = plan.dynlds;
     switch (rpt) {
         case 1: hipLaunchKernelGGL(k_gmres<1>, dim3(P), dim3(GBS), dynlds, st, g); break;
         case 4: hipLaunchKernelGGL(k_gmres<4>, dim3(P), dim3(GBS), dynlds, st, g); break;

@@ -20,6 +20,41 @@ struct GmresParams {
     int64_t maxiters;
     double abstol, reltol;
 };
+
+// Everything one solve's workgroups read: the matrix, vectors, workspaces and exchange words.
+struct GmresArgs {
+    int64_t N;
+    const int32_t* rowptr;
+    const int32_t* col;
+    const double* val;
+    const double* b;
+    double* x;
+    const double* pl;   // nullable: Jacobi / diagonal left preconditioner
+    const double* pr;   // nullable: diagonal right preconditioner
+    double* V;          // (cap + 1) x N basis
+    double* xr;         // N: correction of the current pass (restart)
+    int memory;
+    int restart;
+    int cap;            // size of the LDS bookkeeping (memory, or maxiters when not restarted)
+    int64_t maxiters;
+    double abstol, reltol;
+    int P;
+    uint64_t* red;      // LL all-reduce slots: 2 parities x P x 2 words, then the barrier counter
+    uint32_t* bar;      // barrier counter (red + 4 P)
+    GmresOut* out;
+    int64_t timeout;
+};
+
+// Launch shape of one solve of N rows.
+struct GmresPlan {
+    int P;          // workgroups
+    int rpt;        // rows per thread (1, 4 or 16)
+    int cap;        // Krylov dimension held in LDS
+    size_t dynlds;  // dynamic LDS bytes per workgroup
+};
+
+// 64-bit words of a solve's exchange area (all-reduce slots and barrier counter); zero before use.
+__host__ __device__ inline int64_t gmres_red_words(int P) { return 4 * (int64_t)P + 32; }
 }  // namespace lsx
 
 struct lsx_csr {
@@ -43,5 +78,15 @@ struct lsx_csr {
 namespace lsx {
 int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx, const double* dpl,
                  const double* dpr, hipStream_t st);
+// Workgroups, rows per thread and LDS of a solve; LSX_EINVAL (message in ctx) when it does not fit.
+int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out);
+// Solves of one rows-per-thread class in one launch on `pool` workgroups (at least the widest
+// solve's count): rank r of solve i is work number first[i] + r (first has nitems + 1 entries,
+// first[nitems] = total); workgroups take numbers from *ticket as they start and after each solve,
+// so every solve but the last one started has all its ranks running.  Zeroes each solve's
+// exchange area, *out and *ticket first.  items/first are device copies; dynlds the largest of
+// the solves' plans.
+int gmres_launch_multi(lsx_ctx* ctx, hipStream_t st, const GmresArgs* items, const int* first, int nitems,
+                       int pool, int rpt, size_t dynlds, uint32_t* ticket);
 int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st);
 }  // namespace lsx

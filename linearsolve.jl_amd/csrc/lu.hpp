@@ -16,6 +16,9 @@ constexpr int64_t LGMAX_BYTES = 144 * 1024;  // LDS of the deferred L-interchang
 struct lsx_lu {
     lsx_ctx* ctx = nullptr;
     int64_t n = 0;
+    int64_t cap = 0;                // order the buffers hold (n <= cap; FP64 handles can be reshaped)
+    bool own_F = true;              // F allocated here (false: the caller points F at its memory)
+    bool lookahead = true;          // panel of the next block on stream2 (false: one stream)
     int dtype = LSX_F64;
     int64_t ld = 0;
     hipStream_t stream = nullptr;   // trailing updates and everything outside getrf
@@ -58,7 +61,11 @@ struct lsx_lu {
 };
 
 namespace lsx {
+// Buffers for order lu->cap (0: lu->n).  With own_F false F is not allocated; with lookahead
+// false getrf runs on lu->stream alone and no CU-partitioned streams are made.
 int lu64_alloc(lsx_lu* lu);
+// An FP64 handle takes a new order n <= cap (factors of the previous order are dropped).
+int lu64_reshape(lsx_lu* lu, int64_t n);
 void lu64_free(lsx_lu* lu);
 // Factors lu->F in place (already holding A).  Returns info >= 0 or an lsx_status < 0.
 int lu64_getrf(lsx_lu* lu);

@@ -45,8 +45,8 @@ int lu_ensure_work(lsx_lu* lu, int64_t nrhs) {
     lu->wb = lu->wy = nullptr;
     lu->wcap = 0;
     const int64_t cap = std::max<int64_t>(nrhs, TRHS);
-    LSX_HIP(ctx, hipMalloc(&lu->wb, sizeof(double) * lu->n * cap));
-    LSX_HIP(ctx, hipMalloc(&lu->wy, sizeof(double) * lu->n * cap));
+    LSX_HIP(ctx, hipMalloc(&lu->wb, sizeof(double) * std::max(lu->n, lu->cap) * cap));
+    LSX_HIP(ctx, hipMalloc(&lu->wy, sizeof(double) * std::max(lu->n, lu->cap) * cap));
     lu->wcap = cap;
     return LSX_OK;
 }

@@ -1,18 +1,27 @@
 // serve.hip — continuous-batching request loop for mixed dense / sparse solves on one GPU.
 //
 // Requests are submitted from any host thread into the queue of their lane, in arrival order.
-// A lane dispatches its next request(s) when it is free and their estimated HBM footprint fits
-// what the running requests leave of the budget (admission), and releases the memory when they
-// complete:
-//   batch lane        dense systems with n <= 512: every waiting system that fits goes into one
-//                     persistent batched-LU launch, so requests that arrive while a launch runs
-//                     join the next one (continuous batching);
-//   dense lane        dense systems with n > 512 (panel workgroups wait on each other: at most
-//                     32 of them, one per CU);
-//   large GMRES lane  SimpleGMRES solves that need more than one workgroup (at most 128);
-//   GMRES lanes       SimpleGMRES solves that fit one workgroup (N <= 8192), several at once.
-// The two lanes with cooperating workgroups need at most 160 CUs together, so both always fit
-// on the GPU's 256 once the other (always terminating) kernels drain.
+// Admission: a request starts when its lane has a free worker, its device memory can be carved
+// out of the server's heap (one allocation of the HBM budget made at creation, so admission is
+// the allocation itself and no request calls hipMalloc/hipFree), and the workgroups that wait on
+// each other fit the cooperative budget (below).  Memory and workgroups return when it completes.
+//   batch lane   dense systems with n <= 512: every waiting system that fits the batch arena goes
+//                into one persistent batched-LU launch, so requests that arrive while a launch
+//                runs join the next one (continuous batching);
+//   dense lanes  dense systems with n > 512, one at a time per lane on a reusable LU handle
+//                (grown when a larger order arrives) whose factors live in the request's heap
+//                block;
+//   GMRES lanes  SimpleGMRES solves: every waiting solve of the same rows-per-thread class that
+//                fits goes into one multi-solve launch (continuous batching again) on a pool of
+//                persistent workgroups, so the GMRES lanes together hold at most 3/4 of the CUs.
+// Cooperative budget: the panel kernel of a dense LU and every GMRES solve spread over
+// workgroups that wait for each other, so a solve makes progress only once all its workgroups
+// run.  Within a multi-solve launch workgroups take their numbers in start order, so at most one
+// solve per launch is partly started (the pool is at least as wide as the widest solve, so the
+// other solves' ends fill it up); a lane reserves the largest solve's workgroup count and
+// the reservations of all lanes stay below the CU count minus 16.  Then the partly started
+// solves can never hold every CU, and the other kernels always finish, so every solve starts
+// completely in the end.
 // Lanes run on their own streams and threads, so their work overlaps on the device.  Every
 // kernel is deterministic and a request's arithmetic does not depend on what else is in flight:
 // a result is the same alone and in the mix.  Latency = completion - submission on the serving
@@ -24,7 +33,9 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <climits>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -55,6 +66,7 @@ __device__ __forceinline__ double gauss(uint64_t seed, uint64_t i) {
 struct GenItem {
     int64_t n, aoff, voff;
     uint64_t seed;
+    int64_t ld;  // leading dimension of the system's A
 };
 
 // A = I - g J (J iid N(0,1)/sqrt(n), g ~ U[1e-3, 1e-1]) and b iid N(0,1), for a list of systems
@@ -67,7 +79,7 @@ __global__ void k_gen_dense(const GenItem* __restrict items, double* __restrict 
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n * n;
          idx += (int64_t)gridDim.x * blockDim.x) {
         const int64_t col = idx / n, row = idx - col * n;
-        A[it.aoff + idx] = (row == col ? 1.0 : 0.0) - sc * gauss(it.seed, (uint64_t)idx);
+        A[it.aoff + col * it.ld + row] = (row == col ? 1.0 : 0.0) - sc * gauss(it.seed, (uint64_t)idx);
     }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         b[it.voff + i] = gauss(it.seed ^ 0xB5ull, (uint64_t)i);
@@ -110,19 +122,130 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-enum Lane { LANE_BATCH = 0, LANE_DENSE = 1, LANE_GMRES_BIG = 2, LANE_GMRES = 3, NLANES = 4 };
+enum Lane { LANE_BATCH = 0, LANE_DENSE = 1, LANE_GMRES = 2, NLANES = 3 };
+constexpr int MAXG = 64;           // solves per GMRES launch
+constexpr int COOP_MARGIN = 16;    // CUs never reserved by waiting workgroups
+constexpr size_t HALIGN = 256;     // heap and carve alignment
+
+inline size_t aup(size_t b) { return (b + HALIGN - 1) & ~(HALIGN - 1); }
+
+// Consecutive arrays of one heap block (base 0: measures the block).
+struct Carve {
+    uintptr_t p;
+    template <typename T>
+    T* take(int64_t count) {
+        T* r = reinterpret_cast<T*>(p);
+        p += aup(sizeof(T) * (size_t)std::max<int64_t>(count, 1));
+        return r;
+    }
+};
+
+// First-fit allocator over one device allocation; free ranges kept sorted and coalesced.
+struct Heap {
+    uintptr_t base = 0;
+    size_t size = 0, used = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+
+    void init(void* b, size_t s) {
+        base = reinterpret_cast<uintptr_t>(b);
+        size = s;
+        used = 0;
+        free_.clear();
+        if (s) free_[0] = s;
+    }
+    size_t alloc(size_t bytes) {  // offset, or SIZE_MAX when no free range is large enough
+        bytes = aup(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            used += bytes;
+            return off;
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = aup(bytes);
+        used -= bytes;
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) {
+                off = pv->first;
+                bytes += pv->second;
+                free_.erase(pv);
+            }
+        }
+        if (nx != free_.end() && off + bytes == nx->first) {
+            bytes += nx->second;
+            free_.erase(nx);
+        }
+        free_[off] = bytes;
+    }
+};
 
 struct Job {
     uint64_t ticket = 0;
     lsx_request req{};
     double* x_out = nullptr;
-    size_t footprint = 0;
     int lane = 0;
+    size_t footprint = 0;   // bytes: heap block (dense, GMRES) or batch-arena share (batch)
+    size_t off = 0;         // heap offset
+    int coop = 0;           // workgroups that wait on each other
+    GmresPlan plan{};       // GMRES: launch shape
+    int64_t N = 0, nnz = 0; // GMRES: rows and nonzeros
     bool done = false;
     lsx_result res{};
 };
 
-struct Arena {
+// Heap block of a GMRES request.
+struct GmresBlock {
+    int32_t* rowptr;
+    int32_t* col;
+    double* val;
+    double* b;
+    double* pl;
+    double* x;
+    double* V;
+    double* xr;
+    uint64_t* red;
+    size_t bytes;
+};
+
+GmresBlock gmres_block(uintptr_t base, const Job& j) {
+    Carve c{base};
+    GmresBlock g;
+    g.rowptr = c.take<int32_t>(j.N + 1);
+    g.col = c.take<int32_t>(j.nnz);
+    g.val = c.take<double>(j.nnz);
+    g.b = c.take<double>(j.N);
+    g.pl = c.take<double>(j.N);
+    g.x = c.take<double>(j.N);
+    g.V = c.take<double>((int64_t)(j.plan.cap + 1) * j.N);
+    g.xr = c.take<double>(j.N);
+    g.red = c.take<uint64_t>(gmres_red_words(j.plan.P));
+    g.bytes = c.p - base;
+    return g;
+}
+
+// Heap block of a dense request: the factors (leading dimension pad_ld(n)) and b, solved in place.
+struct DenseBlock {
+    double* F;
+    double* x;
+    size_t bytes;
+};
+
+DenseBlock dense_block(uintptr_t base, int64_t n) {
+    Carve c{base};
+    DenseBlock d;
+    d.F = c.take<double>(pad_ld(n) * n);
+    d.x = c.take<double>(n);
+    d.bytes = c.p - base;
+    return d;
+}
+
+struct Arena {  // batch lane
     double* A = nullptr;
     double* b = nullptr;
     double* x = nullptr;
@@ -134,6 +257,7 @@ struct Arena {
     GenItem* gen = nullptr;
     uint32_t* counter = nullptr;
     int64_t capA = 0, capV = 0, capN = 0;
+    size_t bytes = 0;            // admission share of the budget
     // pinned staging
     int32_t* h32 = nullptr;
     int64_t* h64 = nullptr;
@@ -142,11 +266,30 @@ struct Arena {
     hipStream_t stream = nullptr;
 };
 
+struct DenseLane {
+    lsx_lu* lu = nullptr;        // reusable handle (factors point into the request's block)
+    GenItem* d_gen = nullptr;
+    GenItem* h_gen = nullptr;    // pinned
+};
+
+struct GmresLane {
+    hipStream_t stream = nullptr;
+    GmresArgs* h_items = nullptr;  // pinned
+    int* h_first = nullptr;        // pinned
+    GmresOut* h_out = nullptr;     // pinned
+    GmresArgs* d_items = nullptr;
+    int* d_first = nullptr;
+    GmresOut* d_out = nullptr;
+    uint32_t* d_ticket = nullptr;
+};
+
 }  // namespace
 
 struct lsx_serve {
     lsx_ctx* ctx = nullptr;
-    size_t budget = 0, used = 0, peak = 0;
+    size_t budget = 0, peak = 0, batch_used = 0;
+    int coop_total = 0, coop_used = 0;
+    int gmres_pool = 0;               // workgroups of one GMRES lane's launch (CUs it occupies)
     Clock::time_point t0;
     std::mutex mu;
     std::condition_variable cv;       // workers wait for lane work
@@ -156,169 +299,305 @@ struct lsx_serve {
     uint64_t next_ticket = 1;
     bool stop = false;
     int64_t submitted = 0, completed = 0, batches = 0, batched_reqs = 0, inflight = 0;
+    Heap heap;
+    void* heap_mem = nullptr;
     Arena arena;
+    std::vector<DenseLane> dense;
+    std::vector<GmresLane> gmres;
     std::vector<std::thread> workers;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    void note_use() { peak = std::max(peak, heap.used + batch_used); }
 };
 
 namespace {
 
-size_t footprint(const lsx_request& r, int& lane) {
+// Sizes, lane, footprint and cooperative workgroups of a request; LSX_OK or an error (message set).
+int classify(lsx_serve* s, Job* j) {
+    const lsx_request& r = j->req;
     if (r.kind == 0) {
-        const size_t n = (size_t)r.n;
-        if (r.n <= 512) {
-            lane = LANE_BATCH;
-            return 8 * n * n + 64 * n;
+        const int64_t n = r.n;
+        if (n <= 512) {
+            j->lane = LANE_BATCH;
+            j->footprint = (size_t)(8 * n * n + 64 * n);
+            return LSX_OK;
         }
-        lane = LANE_DENSE;
-        return 16 * n * n + 256 * n + (2 << 20);  // A staging + resident factors and workspaces
+        if (n > (int64_t)PMAX * 512)
+            return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: dense n = %lld exceeds %d", (long long)n, PMAX * 512);
+        j->lane = LANE_DENSE;
+        j->footprint = dense_block(0, n).bytes;
+        j->coop = (int)cdiv(n, 512) + 1;  // panel workgroups, plus the chained solve's running one
+        return LSX_OK;
     }
-    const int64_t N = r.synth ? (int64_t)r.grid_m * r.grid_m : r.n;
-    const int64_t nnz = r.synth ? convdiff_nnz(r.grid_m) : r.nnz;
-    const int64_t mem = std::max<int64_t>(1, std::min<int64_t>(r.memory, r.maxiters));
-    lane = (N <= 8192) ? LANE_GMRES : LANE_GMRES_BIG;
-    return (size_t)(24 * nnz + 8 * N * (mem + 8)) + (1 << 20);
-}
-
-bool fits(lsx_serve* s, size_t f) { return s->used == 0 || s->used + f <= s->budget; }
-
-void reserve_locked(lsx_serve* s, size_t f) {
-    s->used += f;
-    s->peak = std::max(s->peak, s->used);
+    j->lane = LANE_GMRES;
+    j->N = r.synth ? (int64_t)r.grid_m * r.grid_m : r.n;
+    j->nnz = r.synth ? convdiff_nnz(r.grid_m) : r.nnz;
+    if (j->nnz < 0 || j->nnz > INT32_MAX) return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: nnz out of range");
+    const GmresParams p{r.memory, r.restart ? 1 : 0, r.maxiters, r.abstol, r.reltol};
+    int rc = gmres_plan(s->ctx, j->N, p, &j->plan);
+    if (rc) return rc;
+    j->coop = j->plan.P;
+    j->footprint = gmres_block(0, *j).bytes;
+    return LSX_OK;
 }
 
 void finish(lsx_serve* s, Job* j) {
     std::lock_guard<std::mutex> g(s->mu);
     j->res.t_done = s->now();
     j->done = true;
-    s->used -= j->footprint;
+    if (j->lane == LANE_BATCH) s->batch_used -= j->footprint;
+    else s->heap.release(j->off, j->footprint);
     s->completed += 1;
     s->inflight -= 1;
     s->cv.notify_all();
     s->cv_done.notify_all();
 }
 
-// Next request of a lane, once the lane is free and its footprint fits the budget.
-Job* take(lsx_serve* s, int lane) {
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->cv.wait(lk, [&] {
-        return s->stop || (!s->lanes[lane].empty() && fits(s, s->lanes[lane].front()->footprint));
-    });
-    if (s->stop) return nullptr;
-    Job* j = s->lanes[lane].front();
-    s->lanes[lane].pop_front();
-    reserve_locked(s, j->footprint);
+void release_coop(lsx_serve* s, int k) {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->coop_used -= k;
+    s->cv.notify_all();
+}
+
+// ---- dense lanes: one LU at a time per lane (called with s->mu held)
+Job* take_dense(lsx_serve* s) {
+    auto& q = s->lanes[LANE_DENSE];
+    if (q.empty()) return nullptr;
+    Job* j = q.front();
+    if (j->coop > s->coop_total - s->coop_used) return nullptr;
+    const size_t off = s->heap.alloc(j->footprint);
+    if (off == SIZE_MAX) return nullptr;
+    q.pop_front();
+    j->off = off;
+    s->coop_used += j->coop;
+    s->note_use();
     j->res.t_start = s->now();
     return j;
 }
 
-// ---- coop and gmres lanes: one request at a time per worker, through the C-ABI entry points
-void run_dense_single(lsx_serve* s, Job* j) {
+// The lane's handle, grown to order n when needed (its buffers other than the factors: a few MB).
+int dense_handle(lsx_ctx* ctx, DenseLane& L, int64_t n) {
+    if (L.lu && L.lu->cap >= n) return lu64_reshape(L.lu, n);
+    if (L.lu) {
+        lu64_free(L.lu);
+        delete L.lu;
+        L.lu = nullptr;
+    }
+    lsx_lu* lu = new (std::nothrow) lsx_lu();
+    if (!lu) return LSX_ENOMEM;
+    lu->ctx = ctx;
+    lu->n = n;
+    lu->cap = std::max<int64_t>(n, 4096);
+    lu->dtype = LSX_F64;
+    lu->own_F = false;
+    lu->lookahead = false;  // one stream per lane: the lanes overlap with each other instead
+    int rc = lu64_alloc(lu);
+    if (rc == LSX_OK) rc = lu_ensure_work(lu, 1);
+    if (rc != LSX_OK) {
+        lu64_free(lu);
+        delete lu;
+        return rc;
+    }
+    L.lu = lu;
+    return lu64_reshape(lu, n);
+}
+
+void run_dense(lsx_serve* s, DenseLane& L, Job* j) {
     lsx_ctx* ctx = s->ctx;
     const lsx_request& r = j->req;
     const int64_t n = r.n;
-    double *dA = nullptr, *db = nullptr, *dx = nullptr;
-    lsx_lu* lu = nullptr;
-    int rc = LSX_OK;
-    if (hipMalloc(&dA, sizeof(double) * n * n) != hipSuccess || hipMalloc(&db, sizeof(double) * n) != hipSuccess ||
-        hipMalloc(&dx, sizeof(double) * n) != hipSuccess) {
-        rc = LSX_ENOMEM;
-    } else {
+    int rc = dense_handle(ctx, L, n);
+    if (rc == LSX_OK) {
+        lsx_lu* lu = L.lu;
+        const hipStream_t st = lu->stream;
+        const DenseBlock d = dense_block(s->heap.base + j->off, n);
+        lu->F = d.F;
         if (r.synth) {
-            GenItem it{n, 0, 0, r.seed};
-            GenItem* dit = nullptr;
-            if (hipMalloc(&dit, sizeof(GenItem)) != hipSuccess || hipMemcpy(dit, &it, sizeof it, hipMemcpyHostToDevice) != hipSuccess)
-                rc = LSX_EHIP;
-            else {
-                hipLaunchKernelGGL(k_gen_dense, dim3((unsigned)std::min<int64_t>(cdiv(n * n, 256), 2048), 1), dim3(256), 0, 0,
-                                   dit, dA, db);
-                if (hipDeviceSynchronize() != hipSuccess) rc = LSX_EHIP;
-            }
-            (void)hipFree(dit);
-        } else {
-            if (hipMemcpy(dA, r.A, sizeof(double) * n * n, hipMemcpyDefault) != hipSuccess ||
-                hipMemcpy(db, r.b, sizeof(double) * n, hipMemcpyDefault) != hipSuccess)
-                rc = LSX_EHIP;
+            *L.h_gen = GenItem{n, 0, 0, r.seed, lu->ld};
+            if (hipMemcpyAsync(L.d_gen, L.h_gen, sizeof(GenItem), hipMemcpyHostToDevice, st) != hipSuccess) rc = LSX_EHIP;
+            else
+                hipLaunchKernelGGL(k_gen_dense, dim3((unsigned)std::min<int64_t>(cdiv(n * n, 256), 2048), 1), dim3(256), 0,
+                                   st, (const GenItem*)L.d_gen, d.F, d.x);
+        } else if (hipMemcpy2DAsync(d.F, sizeof(double) * lu->ld, r.A, sizeof(double) * n, sizeof(double) * n, n,
+                                    hipMemcpyDefault, st) != hipSuccess ||
+                   hipMemcpyAsync(d.x, r.b, sizeof(double) * n, hipMemcpyDefault, st) != hipSuccess) {
+            rc = LSX_EHIP;
         }
-        if (rc == LSX_OK) rc = lsx_lu_create(ctx, n, LSX_F64, &lu);
         if (rc == LSX_OK) {
-            const int info = lsx_lu_factor(lu, dA, n);
+            const int info = lu64_getrf(lu);  // waits for its stream
             if (info < 0) rc = info;
             else {
                 j->res.info = info;
+                lu->factored = true;
                 if (info > 0) j->res.retcode = LSX_RET_FAILURE;
                 else {
-                    rc = lsx_lu_solve(lu, 1, db, n, dx, n);
+                    rc = lu64_getrs_dev(lu, 1, d.x, n, d.x, n);
                     j->res.retcode = LSX_RET_SUCCESS;
+                    if (rc == LSX_OK &&
+                        (hipMemcpyAsync(j->x_out, d.x, sizeof(double) * n, hipMemcpyDefault, st) != hipSuccess ||
+                         hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t), hipMemcpyDeviceToHost, st) !=
+                             hipSuccess ||
+                         hipStreamSynchronize(st) != hipSuccess))
+                        rc = LSX_EHIP;
+                    if (rc == LSX_OK && lu->hinfo[1]) rc = LSX_ETIMEOUT;
                 }
             }
         }
-        if (rc == LSX_OK && hipMemcpy(j->x_out, dx, sizeof(double) * n, hipMemcpyDefault) != hipSuccess) rc = LSX_EHIP;
+        lu->F = nullptr;
     }
-    if (lu) lsx_lu_destroy(lu);
-    (void)hipFree(dA);
-    (void)hipFree(db);
-    (void)hipFree(dx);
     j->res.status = rc;
 }
 
-void run_gmres(lsx_serve* s, Job* j) {
-    lsx_ctx* ctx = s->ctx;
-    const lsx_request& r = j->req;
-    int rc = LSX_OK;
-    lsx_csr* A = nullptr;
-    int32_t *rp = nullptr, *ci = nullptr;
-    double *va = nullptr, *db = nullptr, *dd = nullptr, *dx = nullptr;
-    const double* b = r.b;
-    const double* pl = r.pl_diag;
-    int64_t N = r.n;
-    if (r.synth) {
-        const int64_t m = r.grid_m;
-        N = m * m;
-        const int64_t nnz = convdiff_nnz(m);
-        if (hipMalloc(&rp, sizeof(int32_t) * (N + 1)) != hipSuccess || hipMalloc(&ci, sizeof(int32_t) * nnz) != hipSuccess ||
-            hipMalloc(&va, sizeof(double) * nnz) != hipSuccess || hipMalloc(&db, sizeof(double) * N) != hipSuccess ||
-            hipMalloc(&dd, sizeof(double) * N) != hipSuccess)
-            rc = LSX_ENOMEM;
-        else {
-            hipLaunchKernelGGL(k_gen_convdiff, dim3((unsigned)std::min<int64_t>(cdiv(N, 256), 4096)), dim3(256), 0, 0,
-                               (int)m, r.pe, r.seed, rp, ci, va, db, dd);
-            if (hipDeviceSynchronize() != hipSuccess) rc = LSX_EHIP;
-            if (rc == LSX_OK) rc = lsx_csr_create(ctx, N, nnz, rp, ci, va, &A);
-            b = db;
-            pl = dd;
-        }
-    } else {
-        rc = lsx_csr_create(ctx, N, r.nnz, r.rowptr, r.colind, r.vals, &A);
-    }
-    if (rc == LSX_OK && hipMalloc(&dx, sizeof(double) * N) != hipSuccess) rc = LSX_ENOMEM;
-    if (rc == LSX_OK) {
-        lsx_gmres_stats st{};
-        rc = lsx_gmres_solve(A, b, dx, r.memory, r.restart, r.abstol, r.reltol, r.maxiters, pl, nullptr, &st);
-        if (rc == LSX_OK) {
-            j->res.retcode = st.retcode;
-            j->res.iters = st.iters;
-            j->res.resid = st.resid;
-            if (hipMemcpy(j->x_out, dx, sizeof(double) * N, hipMemcpyDefault) != hipSuccess) rc = LSX_EHIP;
-        }
-    }
-    if (A) lsx_csr_destroy(A);
-    (void)hipFree(rp);
-    (void)hipFree(ci);
-    (void)hipFree(va);
-    (void)hipFree(db);
-    (void)hipFree(dd);
-    (void)hipFree(dx);
-    j->res.status = rc;
-}
-
-void single_worker(lsx_serve* s, int lane) {
+void dense_worker(lsx_serve* s, int lane) {
     (void)hipSetDevice(s->ctx->device);
-    while (Job* j = take(s, lane)) {
-        if (j->req.kind == 0) run_dense_single(s, j);
-        else run_gmres(s, j);
+    DenseLane& L = s->dense[lane];
+    while (true) {
+        Job* j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || (j = take_dense(s)) != nullptr; });
+            if (!j) return;
+        }
+        run_dense(s, L, j);
+        release_coop(s, j->coop);
         finish(s, j);
+    }
+}
+
+// ---- GMRES lanes: multi-solve launches (called with s->mu held).  The front request must fit;
+// later requests of the same rows-per-thread class join while memory and workgroups allow.
+int take_gmres(lsx_serve* s, std::vector<Job*>& group) {
+    auto& q = s->lanes[LANE_GMRES];
+    if (q.empty()) return 0;
+    Job* f = q.front();
+    const int avail = s->coop_total - s->coop_used;
+    if (f->coop > avail) return 0;
+    size_t off = s->heap.alloc(f->footprint);
+    if (off == SIZE_MAX) return 0;
+    q.pop_front();
+    f->off = off;
+    group.push_back(f);
+    int reserve = f->coop;
+    int64_t total = f->coop;
+    int scanned = 0;
+    for (auto it = q.begin(); it != q.end() && (int)group.size() < MAXG && scanned < 512; ++scanned) {
+        Job* j = *it;
+        if (j->plan.rpt != f->plan.rpt) {
+            ++it;
+            continue;
+        }
+        const int grow = std::max(reserve, j->coop) - reserve;
+        // about two pool turns of work per launch: later arrivals join the next launch soon
+        if (reserve + grow > avail || total >= 2 * (int64_t)s->gmres_pool) break;
+        off = s->heap.alloc(j->footprint);
+        if (off == SIZE_MAX) break;
+        j->off = off;
+        reserve += grow;
+        total += j->coop;
+        group.push_back(j);
+        it = q.erase(it);
+    }
+    const double t = s->now();
+    for (Job* j : group) j->res.t_start = t;
+    s->coop_used += reserve;
+    s->note_use();
+    return reserve;
+}
+
+void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
+    lsx_ctx* ctx = s->ctx;
+    const hipStream_t st = L.stream;
+    const int cnt = (int)group.size();
+    int rc = LSX_OK;
+    int total = 0;
+    size_t lds = 0;
+    std::vector<GmresBlock> blk(cnt);
+    for (int i = 0; i < cnt && rc == LSX_OK; ++i) {
+        Job* j = group[i];
+        const lsx_request& r = j->req;
+        const GmresBlock g = gmres_block(s->heap.base + j->off, *j);
+        blk[i] = g;
+        const int64_t N = j->N;
+        bool has_pl = r.pl_diag != nullptr;
+        if (r.synth) {
+            hipLaunchKernelGGL(k_gen_convdiff, dim3((unsigned)std::min<int64_t>(cdiv(N, 256), 4096)), dim3(256), 0, st,
+                               (int)r.grid_m, r.pe, r.seed, g.rowptr, g.col, g.val, g.b, g.pl);
+            has_pl = true;
+        } else if (hipMemcpyAsync(g.rowptr, r.rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDefault, st) != hipSuccess ||
+                   (j->nnz > 0 && (hipMemcpyAsync(g.col, r.colind, sizeof(int32_t) * j->nnz, hipMemcpyDefault, st) != hipSuccess ||
+                                   hipMemcpyAsync(g.val, r.vals, sizeof(double) * j->nnz, hipMemcpyDefault, st) != hipSuccess)) ||
+                   hipMemcpyAsync(g.b, r.b, sizeof(double) * N, hipMemcpyDefault, st) != hipSuccess ||
+                   (has_pl && hipMemcpyAsync(g.pl, r.pl_diag, sizeof(double) * N, hipMemcpyDefault, st) != hipSuccess)) {
+            rc = LSX_EHIP;
+        }
+        GmresArgs& a = L.h_items[i];
+        a.N = N;
+        a.rowptr = g.rowptr;
+        a.col = g.col;
+        a.val = g.val;
+        a.b = g.b;
+        a.x = g.x;
+        a.pl = has_pl ? g.pl : nullptr;
+        a.pr = nullptr;
+        a.V = g.V;
+        a.xr = g.xr;
+        a.memory = (int)std::max<int64_t>(1, std::min<int64_t>(r.memory, r.maxiters));
+        a.restart = r.restart ? 1 : 0;
+        a.cap = j->plan.cap;
+        a.maxiters = r.maxiters;
+        a.abstol = r.abstol;
+        a.reltol = r.reltol;
+        a.P = j->plan.P;
+        a.red = g.red;
+        a.bar = reinterpret_cast<uint32_t*>(g.red + 4 * (int64_t)j->plan.P);
+        a.out = L.d_out + i;
+        a.timeout = ctx->wall_khz * 20000;  // 20 s
+        L.h_first[i] = total;
+        total += j->plan.P;
+        lds = std::max(lds, j->plan.dynlds);
+    }
+    L.h_first[cnt] = total;
+    if (rc == LSX_OK &&
+        (hipMemcpyAsync(L.d_items, L.h_items, sizeof(GmresArgs) * cnt, hipMemcpyHostToDevice, st) != hipSuccess ||
+         hipMemcpyAsync(L.d_first, L.h_first, sizeof(int) * (cnt + 1), hipMemcpyHostToDevice, st) != hipSuccess))
+        rc = LSX_EHIP;
+    int widest = 0;
+    for (Job* j : group) widest = std::max(widest, j->plan.P);
+    const int pool = std::min(total, std::max(widest, s->gmres_pool));
+    if (rc == LSX_OK) rc = gmres_launch_multi(ctx, st, L.d_items, L.d_first, cnt, pool, group[0]->plan.rpt, lds, L.d_ticket);
+    if (rc == LSX_OK &&
+        hipMemcpyAsync(L.h_out, L.d_out, sizeof(GmresOut) * cnt, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = LSX_EHIP;
+    for (int i = 0; i < cnt && rc == LSX_OK; ++i)
+        if (hipMemcpyAsync(group[i]->x_out, blk[i].x, sizeof(double) * group[i]->N, hipMemcpyDefault, st) != hipSuccess)
+            rc = LSX_EHIP;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == LSX_OK) rc = LSX_EHIP;
+    for (int i = 0; i < cnt; ++i) {
+        Job* j = group[i];
+        j->res.status = rc;
+        if (rc == LSX_OK) {
+            if (L.h_out[i].err) j->res.status = LSX_ETIMEOUT;
+            j->res.retcode = L.h_out[i].retcode;
+            j->res.iters = L.h_out[i].iters;
+            j->res.resid = L.h_out[i].resid;
+        }
+    }
+}
+
+void gmres_worker(lsx_serve* s, int lane) {
+    (void)hipSetDevice(s->ctx->device);
+    GmresLane& L = s->gmres[lane];
+    while (true) {
+        std::vector<Job*> group;
+        int reserve = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || (reserve = take_gmres(s, group)) > 0; });
+            if (group.empty()) return;
+        }
+        run_gmres_group(s, L, group);
+        release_coop(s, reserve);
+        for (Job* j : group) finish(s, j);
     }
 }
 
@@ -330,24 +609,25 @@ void batch_worker(lsx_serve* s) {
         std::vector<Job*> group;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv.wait(lk, [&] {
-                return s->stop || (!s->lanes[LANE_BATCH].empty() && fits(s, s->lanes[LANE_BATCH].front()->footprint));
-            });
+            s->cv.wait(lk, [&] { return s->stop || !s->lanes[LANE_BATCH].empty(); });
             if (s->stop) return;
             int64_t ua = 0, uv = 0;
+            size_t ub = 0;
             const double t = s->now();
             while (!s->lanes[LANE_BATCH].empty()) {
                 Job* j = s->lanes[LANE_BATCH].front();
                 const int64_t n = j->req.n;
                 if ((int64_t)group.size() + 1 > a.capN || ua + n * n > a.capA || uv + n > a.capV) break;
-                if (!group.empty() && !fits(s, j->footprint)) break;
+                if (!group.empty() && ub + j->footprint > a.bytes) break;
                 s->lanes[LANE_BATCH].pop_front();
-                reserve_locked(s, j->footprint);
                 j->res.t_start = t;
                 group.push_back(j);
                 ua += n * n;
                 uv += n;
+                ub += j->footprint;
             }
+            s->batch_used += ub;
+            s->note_use();
         }
         const int64_t cnt = (int64_t)group.size();
         // metadata: order by decreasing n (longest first), offsets in arrival order
@@ -362,7 +642,7 @@ void batch_worker(lsx_serve* s) {
             a.h32[a.capN + i] = (int32_t)r.n;
             a.h64[i] = ao;
             a.h64[a.capN + i] = vo;
-            if (r.synth) a.hgen[ngen++] = GenItem{r.n, ao, vo, r.seed};
+            if (r.synth) a.hgen[ngen++] = GenItem{r.n, ao, vo, r.seed, r.n};
             else if (hipMemcpyAsync(a.A + ao, r.A, sizeof(double) * r.n * r.n, hipMemcpyDefault, a.stream) != hipSuccess ||
                      hipMemcpyAsync(a.b + vo, r.b, sizeof(double) * r.n, hipMemcpyDefault, a.stream) != hipSuccess)
                 rc = LSX_EHIP;
@@ -407,6 +687,7 @@ void batch_worker(lsx_serve* s) {
 }
 
 int arena_alloc(lsx_ctx* ctx, Arena& a, size_t bytes) {
+    a.bytes = bytes;
     a.capA = std::max<int64_t>(512 * 512, (int64_t)(bytes / sizeof(double)));
     a.capV = 4 << 20;
     a.capN = 16384;
@@ -444,6 +725,55 @@ void arena_free(Arena& a) {
     if (a.stream) (void)hipStreamDestroy(a.stream);
 }
 
+int lanes_alloc(lsx_serve* s, int nd, int ng) {
+    s->dense.resize(nd);
+    s->gmres.resize(ng);
+    for (auto& L : s->dense)
+        if (hipMalloc(&L.d_gen, sizeof(GenItem)) != hipSuccess ||
+            hipHostMalloc(&L.h_gen, sizeof(GenItem), hipHostMallocDefault) != hipSuccess)
+            return fail(s->ctx, LSX_ENOMEM, "lsx_serve_create: cannot allocate a dense lane");
+    for (auto& L : s->gmres)
+        if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipHostMalloc(&L.h_items, sizeof(GmresArgs) * MAXG, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&L.h_first, sizeof(int) * (MAXG + 1), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&L.h_out, sizeof(GmresOut) * MAXG, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&L.d_items, sizeof(GmresArgs) * MAXG) != hipSuccess ||
+            hipMalloc(&L.d_first, sizeof(int) * (MAXG + 1)) != hipSuccess ||
+            hipMalloc(&L.d_out, sizeof(GmresOut) * MAXG) != hipSuccess || hipMalloc(&L.d_ticket, 256) != hipSuccess)
+            return fail(s->ctx, LSX_ENOMEM, "lsx_serve_create: cannot allocate a GMRES lane");
+    return LSX_OK;
+}
+
+void lanes_free(lsx_serve* s) {
+    for (auto& L : s->dense) {
+        if (L.lu) {
+            lu64_free(L.lu);
+            delete L.lu;
+        }
+        (void)hipFree(L.d_gen);
+        if (L.h_gen) (void)hipHostFree(L.h_gen);
+    }
+    for (auto& L : s->gmres) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        if (L.h_items) (void)hipHostFree(L.h_items);
+        if (L.h_first) (void)hipHostFree(L.h_first);
+        if (L.h_out) (void)hipHostFree(L.h_out);
+        (void)hipFree(L.d_items);
+        (void)hipFree(L.d_first);
+        (void)hipFree(L.d_out);
+        (void)hipFree(L.d_ticket);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    s->dense.clear();
+    s->gmres.clear();
+}
+
+void server_free(lsx_serve* s) {
+    lanes_free(s);
+    arena_free(s->arena);
+    (void)hipFree(s->heap_mem);
+}
+
 }  // namespace
 
 extern "C" {
@@ -456,19 +786,28 @@ int lsx_serve_create(lsx_ctx* ctx, size_t hbm_budget, int32_t nstreams, lsx_serv
     if (!s) return LSX_ENOMEM;
     s->ctx = ctx;
     s->t0 = Clock::now();
+    s->budget = hbm_budget;
+    s->coop_total = std::max(64, ctx->num_cus - COOP_MARGIN);
     const size_t arena_bytes = std::min<size_t>(hbm_budget / 4, (size_t)2 << 30);
+    const size_t heap_bytes = (hbm_budget - arena_bytes) & ~(HALIGN - 1);
     int rc = arena_alloc(ctx, s->arena, arena_bytes);
+    if (rc == LSX_OK && hipMalloc(&s->heap_mem, heap_bytes) != hipSuccess)
+        rc = fail(ctx, LSX_ENOMEM, "lsx_serve_create: cannot allocate a %zu-byte request heap", heap_bytes);
+    const int lanes = std::max(2, std::min<int>(nstreams, 8));
+    const int nd = lanes / 2, ng = lanes - nd;
+    if (rc == LSX_OK) rc = lanes_alloc(s, nd, ng);
+    // the GMRES lanes together occupy at most 3/4 of the CUs (one 135 KB-LDS workgroup each), so
+    // the dense and batched lanes' kernels always find CUs
+    s->gmres_pool = std::max(1, ctx->num_cus * 3 / 4 / ng);
     if (rc) {
-        arena_free(s->arena);
+        server_free(s);
         delete s;
         return rc;
     }
-    s->budget = hbm_budget - arena_bytes;
-    const int ng = std::max(1, std::min<int>(nstreams, 8));
+    s->heap.init(s->heap_mem, heap_bytes);
     s->workers.emplace_back(batch_worker, s);
-    s->workers.emplace_back(single_worker, s, (int)LANE_DENSE);
-    s->workers.emplace_back(single_worker, s, (int)LANE_GMRES_BIG);
-    for (int i = 0; i < ng; ++i) s->workers.emplace_back(single_worker, s, (int)LANE_GMRES);
+    for (int i = 0; i < nd; ++i) s->workers.emplace_back(dense_worker, s, i);
+    for (int i = 0; i < ng; ++i) s->workers.emplace_back(gmres_worker, s, i);
     *out = s;
     return LSX_OK;
 }
@@ -481,8 +820,11 @@ int lsx_serve_destroy(lsx_serve* s) {
     }
     s->cv.notify_all();
     for (auto& t : s->workers) t.join();
-    arena_free(s->arena);
-    for (auto& kv : s->jobs) delete kv.second;
+    {
+        DeviceGuard g(s->ctx);
+        server_free(s);
+    }
+    for (auto& kv : s->jobs) delete kv.second;  // every unpolled request, queued ones included
     delete s;
     return LSX_OK;
 }
@@ -493,9 +835,10 @@ int lsx_serve_submit(lsx_serve* s, const lsx_request* req, double* x_out, uint64
         if (req->n < 1 || req->dtype != LSX_F64 || (!req->synth && (!req->A || !req->b)))
             return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: bad dense request");
     } else if (req->kind == 1) {
-        if (req->synth ? req->grid_m < 2 : (req->n < 1 || !req->rowptr || !req->b))
+        if (req->synth ? req->grid_m < 2 : (req->n < 1 || !req->rowptr || !req->b || (req->nnz > 0 && (!req->colind || !req->vals))))
             return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: bad sparse request");
-        if (req->memory < 1) return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: memory >= 1");
+        if (req->memory < 1 || req->maxiters < 0)
+            return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: memory >= 1 and maxiters >= 0");
     } else {
         return fail(s->ctx, LSX_EINVAL, "lsx_serve_submit: kind must be 0 or 1");
     }
@@ -503,11 +846,16 @@ int lsx_serve_submit(lsx_serve* s, const lsx_request* req, double* x_out, uint64
     if (!j) return LSX_ENOMEM;
     j->req = *req;
     j->x_out = x_out;
-    j->footprint = footprint(*req, j->lane);
-    std::lock_guard<std::mutex> g(s->mu);
-    if (j->footprint > s->budget) {
+    int rc = classify(s, j);
+    if (rc) {
         delete j;
-        return fail(s->ctx, LSX_ENOMEM, "lsx_serve_submit: request needs more HBM than the whole budget");
+        return rc;
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    const size_t limit = j->lane == LANE_BATCH ? s->arena.bytes : s->heap.size;
+    if (j->footprint > limit || j->coop > s->coop_total) {
+        delete j;
+        return fail(s->ctx, LSX_ENOMEM, "lsx_serve_submit: request needs more HBM (or workgroups) than the whole budget");
     }
     j->ticket = s->next_ticket++;
     j->res.status = LSX_EAGAIN;

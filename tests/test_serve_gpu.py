@@ -149,3 +149,50 @@ def test_budget_admission(lsx):
     with pytest.raises(lsx.LSXError):
         srv.submit_dense(np.eye(3000), np.ones(3000))
     srv.close()
+
+
+def test_serving_equals_direct_solvers(lsx):
+    # The serving lanes run the same kernels as the plain entry points: the dense lane on a
+    # reshaped, reused LU handle (one stream) and the GMRES lane in a multi-solve launch.  Results
+    # must equal LUFactorization / SimpleGMRES solves of the same inputs bit for bit.
+    from linearsolve_jl_amd import problems as P
+    dense = [P.dense_block(n, seed=n + 3) for n in (1500, 600, 900)]  # handle grows, then reshapes
+    grids = [(100, 3.0), (140, 1.0), (60, 2.0), (130, 0.5)]             # 2, 3, 1, 3 workgroups
+    sparse = []
+    for m, pe in grids:
+        rp, ci, v = P.convection_diffusion(m, pe)
+        N = m * m
+        sparse.append((lsx.CSRMatrix(rp, ci, v, (N, N)), P.rng_for(13, m).standard_normal(N), np.full(N, 4.0)))
+    srv = lsx.Server(hbm_budget=8 << 30)
+    td = [srv.submit_dense(A, b) for A, b in dense]
+    ts = [srv.submit_gmres(A, b, pl=d, memory=30, reltol=1e-8, maxiters=400) for A, b, d in sparse]
+    res = wait_all(srv, td + ts)
+    srv.close()
+    for (A, b), t in zip(dense, td):
+        x = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XLUFactorization()).u
+        np.testing.assert_array_equal(res[t]["x"], x)
+    for (A, b, d), t in zip(sparse, ts):
+        sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XSimpleGMRES(memory=30), Pl=d, reltol=1e-8, maxiters=400)
+        np.testing.assert_array_equal(res[t]["x"], sol.u)
+        assert res[t]["iters"] == sol.iters and int(res[t]["retcode"]) == int(sol.retcode)
+
+
+def test_gmres_multi_launch_many_requests(lsx):
+    # more GMRES requests than one launch holds, of several rows-per-thread classes: all complete,
+    # each converged, and each identical to the same request served alone
+    from linearsolve_jl_amd import problems as P
+    cases = [(m, 0.5 + (i % 5)) for i, m in enumerate([12, 20, 35, 50, 70, 95, 110, 64] * 9)]
+    srv = lsx.Server(hbm_budget=4 << 30)
+    ts = [srv.submit_gmres_synth(m, pe, seed=i, reltol=1e-8, maxiters=3000) for i, (m, pe) in enumerate(cases)]
+    res = wait_all(srv, ts, timeout=300.0)
+    for t in ts:
+        assert res[t]["status"] == 0 and res[t]["retcode"] == lsx.ReturnCode.Success
+        assert res[t]["resid"] <= 1e-6
+    srv.close()
+    for i in (0, 5, 6, 71):
+        m, pe = cases[i]
+        srv1 = lsx.Server(hbm_budget=4 << 30)
+        t1 = srv1.submit_gmres_synth(m, pe, seed=i, reltol=1e-8, maxiters=3000)
+        r1 = wait_all(srv1, [t1])[t1]
+        np.testing.assert_array_equal(r1["x"], res[ts[i]]["x"])
+        srv1.close()
