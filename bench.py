@@ -97,8 +97,8 @@ def cpu_baseline(A, B, n, nrhs):
     t = time.perf_counter() - t0
     flops = luflop(ns) + 2.0 * ns * ns * nrhs
     return dict(value=flops / t / 1e12, unit="TFLOP/s", cores=int(cores), kind="reference",
-                sample=f"one dgetrf+dgetrs ({nrhs} RHS) at n={ns} of the same matrix, OpenBLAS "
-                       f"{getattr(scipy, '__version__', '?')} via scipy (LAPACK, the library "
+                sample=f"one dgetrf+dgetrs ({nrhs} RHS) at n={ns} of the same matrix, scipy "
+                       f"{getattr(scipy, '__version__', '?')}'s OpenBLAS (LAPACK, the library "
                        f"LUFactorization calls; julia is not installed), {t:.2f} s"), (piv + 1 if ns == n else None)
 
 
@@ -212,31 +212,49 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     x = torch.empty_like(b)
     torch.cuda.synchronize()
     log("batched: systems ready, creating the batch handle")
-    hb = ctypes.c_void_p()
-    ctx.check(L.lsx_batch_create(ctx.handle, count, sizes.ctypes.data, _abi.LSX_F64, ctypes.byref(hb)),
-              "lsx_batch_create")
+    table = lsx.load_tile_table()  # autotuned size-binned panel widths (tile_table.json)
     info = np.zeros(count, dtype=np.int64)
-
-    def batched():
-        return ctx.check(L.lsx_batch_factor_solve(hb, A.data_ptr(), b.data_ptr(), x.data_ptr(),
-                                                  info.ctypes.data, None), "lsx_batch_factor_solve")
-
-    batched()
-    log("batched: first launch done")
-    tb, nfail = timed(batched, reps)
-    L.lsx_batch_destroy(hb)
-    log(f"batched: {reps} timed launches done, {tb * 1e3:.1f} ms")
-    # backward error on a sample of systems (host FP64)
     rs = np.random.default_rng(7)
     pick = rs.choice(count, size=min(200, count), replace=False)
-    worst = 0.0
-    for i in pick:
-        n = int(sizes[i])
-        Ai = A[aoff[i]:aoff[i] + n * n].view(n, n).t().cpu().numpy()
-        bi = b[voff[i]:voff[i] + n].cpu().numpy()
-        xi = x[voff[i]:voff[i] + n].cpu().numpy()
-        r = bi - Ai @ xi
-        worst = max(worst, np.abs(r).max() / (np.abs(Ai).sum(axis=1).max() * np.abs(xi).max() + np.abs(bi).max()))
+
+    def sampled_backward_error():
+        worst = 0.0
+        for i in pick:
+            n = int(sizes[i])
+            Ai = A[aoff[i]:aoff[i] + n * n].view(n, n).t().cpu().numpy()
+            bi = b[voff[i]:voff[i] + n].cpu().numpy()
+            xi = x[voff[i]:voff[i] + n].cpu().numpy()
+            r = bi - Ai @ xi
+            worst = max(worst, np.abs(r).max() / (np.abs(Ai).sum(axis=1).max() * np.abs(xi).max() + np.abs(bi).max()))
+        return worst
+
+    def run(dtype):
+        hb = ctypes.c_void_p()
+        ctx.check(L.lsx_batch_create(ctx.handle, count, sizes.ctypes.data, dtype, ctypes.byref(hb)), "lsx_batch_create")
+        if table:
+            up = np.array([u for u, _ in table], dtype=np.int32)
+            bw = np.array([w for _, w in table], dtype=np.int32)
+            ctx.check(L.lsx_batch_set_tiles(hb, len(table), up.ctypes.data, bw.ctypes.data), "lsx_batch_set_tiles")
+
+        def batched():
+            return ctx.check(L.lsx_batch_factor_solve(hb, A.data_ptr(), b.data_ptr(), x.data_ptr(),
+                                                      info.ctypes.data, None), "lsx_batch_factor_solve")
+
+        batched()
+        t, nf = timed(batched, reps)
+        iters = np.zeros(count, dtype=np.int32)
+        berr = np.zeros(count, dtype=np.float64)
+        ctx.check(L.lsx_batch_stats(hb, iters.ctypes.data, berr.ctypes.data), "lsx_batch_stats")
+        L.lsx_batch_destroy(hb)
+        return t, nf, iters, berr
+
+    t32, nf32, it32, be32 = run(_abi.LSX_F32)
+    worst32 = sampled_backward_error()
+    log(f"batched: FP32 + FP64 refinement {t32 * 1e3:.1f} ms, refinement steps max {int(it32.max())}, "
+        f"sampled backward error {worst32:.2e}")
+    tb, nfail, _, _ = run(_abi.LSX_F64)
+    log(f"batched: {reps} timed FP64 launches done, {tb * 1e3:.1f} ms")
+    worst = sampled_backward_error()
     log(f"batched: sampled backward error {worst:.2e}")
     # per-system loop of the plain FP64 GPU LU (handles made beforehand, outside the timing)
     loop_idx = rs.choice(count, size=min(loop_sample, count), replace=False)
@@ -266,7 +284,12 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
                backward_error_max_sampled=worst,
                per_system_loop=dict(solves_per_s=round(1.0 / tl, 1), sample=len(loop_idx),
                                     kind="lsx_lu_factor + lsx_lu_solve per system, handles reused"),
-               speedup_vs_loop=round((count / tb) * tl, 2))
+               speedup_vs_loop=round((count / tb) * tl, 2),
+               tile_table=table if table else "library default",
+               fp32_refined=dict(ms=round(t32 * 1e3, 3), solves_per_s=round(count / t32, 1), failed=int(nf32),
+                                 refinement_steps_max=int(it32.max()), refinement_steps_mean=round(float(it32.mean()), 2),
+                                 backward_error_max=float(be32.max()), backward_error_max_sampled=worst32,
+                                 speedup_vs_loop=round((count / t32) * tl, 2)))
     log(f"batched: {tb * 1e3:.1f} ms, {count / tb:.0f} solves/s, loop {1 / tl:.0f} solves/s, "
         f"speedup {res['speedup_vs_loop']}x, bwd {worst:.2e}")
     del A, b, x
