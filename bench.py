@@ -14,6 +14,8 @@ Beside it, in the same line (each can be switched off with --sections):
            {1e2, 1e4, 1e6}, one right-hand side: FP32 and BF16 factorization + FP64 refinement
            (IR, then GMRES-IR where IR stalls) to backward error <= 1e-13, against the plain FP64
            LU of the same system;
+  gmres    SimpleGMRES(30), Jacobi, on the 1024 x 1024 convection-diffusion grid (Pe = 5), 300
+           Arnoldi steps: us per step and algorithmic HBM GB/s; the CPU restatement on 256 x 256;
   batched  20,000 systems A_i = I - g_i J_i, n_i uniform in [8, 512], one right-hand side each,
            one persistent launch, against a loop of plain per-system GPU LU solves (timed on a
            sample of the same systems);
@@ -297,6 +299,73 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     return res
 
 
+def gmres_step_bytes(N, nnz, iters, memory):
+    """HBM bytes of `iters` Arnoldi steps of restarted GMRES(memory) with a diagonal left
+    preconditioner, step k of a cycle.  Returns (convention, minimum):
+      convention  (128 + 40 k) N, the per-step figure of the unfused algorithm (SpMV ~80 N,
+                  Jacobi 24 N, norm 8 N, next basis vector 16 N, MGS 40 k N: dot and axpy each
+                  read the basis vector and q, and the axpy writes q);
+      minimum     what the fused kernel must move: the CSR (12 nnz + 4 N), the gathered basis
+                  vector (8 N), the diagonal (8 N), each of the k basis vectors once (8 k N: q stays
+                  on chip), the next basis vector written (8 N)."""
+    conv = mini = 0.0
+    for i in range(iters):
+        k = i % memory + 1
+        conv += (128.0 + 40.0 * k) * N
+        mini += 12.0 * nnz + 4.0 * N + 8.0 * N + 8.0 * N + 8.0 * k * N + 8.0 * N
+    return conv, mini
+
+
+def run_gmres(lsx, _abi, ctx, L, m, pe, memory, maxiters, cpu_m):
+    """One SimpleGMRES(memory) solve with the Jacobi preconditioner on the m x m convection-
+    diffusion grid (inputs resident in HBM), and the same solver's CPU restatement (the oracle,
+    one thread) on a smaller grid, both for `maxiters` Arnoldi steps at most."""
+    import torch
+    P = lsx.problems
+    out = {}
+    for side in (m, cpu_m):
+        rp, ci, v = P.convection_diffusion(side, pe)
+        N = side * side
+        b = P.rng_for(12, side).standard_normal(N)
+        A = ctypes.c_void_p()
+        ctx.check(L.lsx_csr_create(ctx.handle, N, len(v), rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                                   ctypes.byref(A)), "lsx_csr_create")
+        db = torch.from_numpy(b).cuda()
+        dx = torch.empty_like(db)
+        dpl = torch.full((N,), 4.0, dtype=torch.float64, device="cuda")
+        st = _abi.GmresStats()
+        tol = float(np.sqrt(np.finfo(np.float64).eps))
+
+        def solve():
+            ctx.check(L.lsx_gmres_solve(A, db.data_ptr(), dx.data_ptr(), memory, 1, tol, 1e-8, maxiters,
+                                        dpl.data_ptr(), None, ctypes.byref(st)), "lsx_gmres_solve")
+
+        t, _ = timed(solve, 2)
+        L.lsx_csr_destroy(A)
+        it = int(st.iters)
+        conv, mini = gmres_step_bytes(N, len(v), it, memory)
+        ent = dict(grid=f"{side}x{side}", N=N, nnz=len(v), iterations=it, retcode=int(st.retcode),
+                   resid=float(st.resid), ms=round(t * 1e3, 3), us_per_step=round(t / max(it, 1) * 1e6, 2),
+                   bytes_convention=conv, bytes_minimum=mini,
+                   achieved_GBs_convention=round(conv / t / 1e9, 1), achieved_GBs_minimum=round(mini / t / 1e9, 1),
+                   hbm_frac_minimum=round(mini / t / 1e9 / HBM_PEAK_GBS, 4))
+        if side == cpu_m:
+            sys.path.insert(0, ROOT)
+            from oracle import oracle as O  # the CPU restatement, timed as the baseline only
+            t0 = time.perf_counter()
+            r = O.gmres(rp, ci, v, b, memory=memory, restart=True, abstol=tol, reltol=1e-8, maxiters=maxiters,
+                        pl=np.full(N, 4.0))
+            tc = time.perf_counter() - t0
+            ent["cpu_oracle"] = dict(ms=round(tc * 1e3, 3), iterations=int(r["iters"]), cores=1, kind="port",
+                                     us_per_step=round(tc / max(int(r["iters"]), 1) * 1e6, 2))
+        out[f"m={side}"] = ent
+        log(f"gmres: grid {side}^2, {it} steps in {t * 1e3:.1f} ms ({ent['us_per_step']} us/step), "
+            f"{ent['achieved_GBs_minimum']} GB/s (fused minimum bytes)" + (f"; CPU oracle {ent['cpu_oracle']['ms']:.0f} ms"
+                                                         if "cpu_oracle" in ent else ""))
+        del db, dx, dpl
+    return dict(memory=memory, maxiters=maxiters, pe=pe, precond="Jacobi (diagonal 4)", runs=out)
+
+
 def _submit_synth(srv, req, gmres_maxiters):
     if req["kind"] == "dense":
         return srv.submit_dense_synth(req["n"], req["seed"])
@@ -384,8 +453,8 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="mixed,batched,serving",
-                    help="comma list of extra sections beside the FP64 LU line (mixed, batched, serving; '' = none)")
+    ap.add_argument("--sections", default="mixed,gmres,batched,serving",
+                    help="comma list of extra sections beside the FP64 LU line (mixed, gmres, batched, serving; '' = none)")
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
@@ -525,6 +594,8 @@ def main():
     if "mixed" in sections:
         result["mixed"] = dict(n=args.mixed_n, nrhs=1, tol=1e-13,
                                runs=run_mixed(lsx, _abi, ctx, L, args.mixed_n, (1e2, 1e4, 1e6), 2))
+    if "gmres" in sections:
+        result["gmres"] = run_gmres(lsx, _abi, ctx, L, 1024, 5.0, 30, 300, 256)
     if "batched" in sections:
         result["batched"] = run_batched(lsx, _abi, ctx, L, args.batch_count, 3, 300)
     if "serving" in sections:

@@ -30,6 +30,7 @@ namespace lsx {
 
 constexpr int GBS = 512;            // threads per workgroup
 constexpr int GNW = GBS / 64;
+constexpr int GLEAD = 16;           // from this many workgroups on, all-reduces go through workgroup 0
 
 struct GShared {
     double wsum[2][GNW];
@@ -76,26 +77,43 @@ struct GmresCtx {
             st_relaxed_u64(slot + 2 * w, ll_pack((uint32_t)__double2loint(s), seq));
             st_relaxed_u64(slot + 2 * w + 1, ll_pack((uint32_t)__double2hiint(s), seq));
         }
-        if (tid < 2 * a.P) {
-            uint64_t x = ld_relaxed_u64(slot + tid);
-            if ((uint32_t)(x >> 32) != seq) {
-                const int64_t t0 = wall_clock64();
-                while (true) {
-                    x = ld_relaxed_u64(slot + tid);
-                    if ((uint32_t)(x >> 32) == seq) break;
-                    if (wall_clock64() - t0 > a.timeout) {
-                        atomicOr((unsigned long long*)&a.out->err, 1ull);
-                        break;
-                    }
+        // Small P: every workgroup reads all P partials and sums them.  From GLEAD workgroups on,
+        // workgroup 0 alone reads them, sums them in the same order and publishes the total, so
+        // the others poll two words instead of 2P (the polling traffic grows as P^2 otherwise).
+        const bool lead = a.P >= GLEAD;
+        if (!lead || w == 0) {
+            if (tid < 2 * a.P) sh.rbuf[par][tid] = poll(slot + tid, seq);
+            __syncthreads();
+            double tot = 0.0;
+            for (int p = 0; p < a.P; ++p)
+                tot += __hiloint2double((int)sh.rbuf[par][2 * p + 1], (int)sh.rbuf[par][2 * p]);
+            if (lead && tid == 0) {
+                uint64_t* res = a.red + 4 * (int64_t)a.P + 8 + 2 * par;
+                st_relaxed_u64(res, ll_pack((uint32_t)__double2loint(tot), seq));
+                st_relaxed_u64(res + 1, ll_pack((uint32_t)__double2hiint(tot), seq));
+            }
+            return tot;
+        }
+        if (tid < 2) sh.rbuf[par][tid] = poll(a.red + 4 * (int64_t)a.P + 8 + 2 * par + tid, seq);
+        __syncthreads();
+        return __hiloint2double((int)sh.rbuf[par][1], (int)sh.rbuf[par][0]);
+    }
+
+    // Payload of a flag-in-word message once it carries sequence number seq (bounded wait).
+    __device__ uint32_t poll(const uint64_t* word, uint32_t seq) const {
+        uint64_t x = ld_relaxed_u64(word);
+        if ((uint32_t)(x >> 32) != seq) {
+            const int64_t t0 = wall_clock64();
+            while (true) {
+                x = ld_relaxed_u64(word);
+                if ((uint32_t)(x >> 32) == seq) break;
+                if (wall_clock64() - t0 > a.timeout) {
+                    atomicOr((unsigned long long*)&a.out->err, 1ull);
+                    break;
                 }
             }
-            sh.rbuf[par][tid] = (uint32_t)x;
         }
-        __syncthreads();
-        double tot = 0.0;
-        for (int p = 0; p < a.P; ++p)
-            tot += __hiloint2double((int)sh.rbuf[par][2 * p + 1], (int)sh.rbuf[par][2 * p]);
-        return tot;
+        return (uint32_t)x;
     }
 
     // Orders this workgroup's global writes before every workgroup's later reads.
@@ -523,7 +541,7 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     g.reltol = p.reltol;
     g.P = P;
     g.red = A->red;
-    g.bar = reinterpret_cast<uint32_t*>(A->red + 4 * PMAX_G);
+    g.bar = reinterpret_cast<uint32_t*>(A->red + 4 * (int64_t)P);
     g.out = A->out;
     g.timeout = ctx->wall_khz * 20000;  // 20 s
     const size_t dynlds = plan.dynlds;

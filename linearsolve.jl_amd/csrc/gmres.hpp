@@ -39,7 +39,8 @@ struct GmresArgs {
     int64_t maxiters;
     double abstol, reltol;
     int P;
-    uint64_t* red;      // LL all-reduce slots: 2 parities x P x 2 words, then the barrier counter
+    uint64_t* red;      // LL all-reduce slots: 2 parities x P x 2 words; word 4 P holds the barrier
+                        // counter, words 4 P + 8 ... 4 P + 11 the totals workgroup 0 publishes
     uint32_t* bar;      // barrier counter (red + 4 P)
     GmresOut* out;
     int64_t timeout;
@@ -53,7 +54,8 @@ struct GmresPlan {
     size_t dynlds;  // dynamic LDS bytes per workgroup
 };
 
-// 64-bit words of a solve's exchange area (all-reduce slots and barrier counter); zero before use.
+// 64-bit words of a solve's exchange area (all-reduce slots, barrier counter, published totals);
+// zero before use.
 __host__ __device__ inline int64_t gmres_red_words(int P) { return 4 * (int64_t)P + 32; }
 }  // namespace lsx
 
