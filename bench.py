@@ -34,9 +34,10 @@ import numpy as np
 
 # The serving lanes (batched, 2 dense, 2 GMRES) and the LU's lookahead streams each want a
 # hardware queue of their own: streams that share one run one after the other, so a dense
-# lane's short kernels would wait behind a whole GMRES launch.  HIP's default is 4 per process;
+# lane's short kernels would wait behind a whole GMRES launch.  HIP's default is 4 per process
+# (the sections before serving create and drop further streams, CU-masked ones among them);
 # it has to be set before the runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
