@@ -183,11 +183,19 @@ int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, 
         const int nr = (int)std::min<int64_t>(TRHS, nrhs - c0);
         const uint32_t s1 = lu->tseq++, s2 = lu->tseq++;
         const int pi = lu->prof.begin(lu->stream, PK_TRSV, 2.0 * n * n * nr, sizeof(T) * (double)n * n);
-        hipLaunchKernelGGL((k_trsv<T, true>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DL,
-                           dB + c0 * ldb, ldb, lu->wy, n, nr, lu->tmsg, s1, timeout, lu->dinfo + 1);
-        hipLaunchKernelGGL((k_trsv<T, false>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DU,
-                           (const double*)lu->wy, n, dX + c0 * ldx, ldx, nr, lu->tmsg, s2, timeout,
-                           lu->dinfo + 1);
+        if (nr == 1) {
+            hipLaunchKernelGGL((k_trsv<T, true, 1>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DL,
+                               dB + c0 * ldb, ldb, lu->wy, n, nr, lu->tmsg, s1, timeout, lu->dinfo + 1);
+            hipLaunchKernelGGL((k_trsv<T, false, 1>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DU,
+                               (const double*)lu->wy, n, dX + c0 * ldx, ldx, nr, lu->tmsg, s2, timeout,
+                               lu->dinfo + 1);
+        } else {
+            hipLaunchKernelGGL((k_trsv<T, true, TRHS>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm,
+                               DL, dB + c0 * ldb, ldb, lu->wy, n, nr, lu->tmsg, s1, timeout, lu->dinfo + 1);
+            hipLaunchKernelGGL((k_trsv<T, false, TRHS>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm,
+                               DU, (const double*)lu->wy, n, dX + c0 * ldx, ldx, nr, lu->tmsg, s2, timeout,
+                               lu->dinfo + 1);
+        }
         lu->prof.end(lu->stream, pi);
     }
     LSX_HIP(ctx, hipGetLastError());

@@ -596,28 +596,40 @@ __global__ __launch_bounds__(64) void k_diag_inv(const T* __restrict F, int64_t 
 // k-quarters of partial sums per row), the message reads of the next block issued before the
 // current one is used, then applies the inverse of its diagonal block.
 constexpr int TMW = 2 * DB * TRHS;  // message words of one block
-template <typename T, bool LOWER>
+// NR: right-hand-side slots handled (1, or TRHS for nrhs > 1); the arithmetic of each value is
+// the same for both.  The block's right-hand side is read before the chain reaches the block.
+template <typename T, bool LOWER, int NR>
 __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld, int64_t n,
                                               const int* __restrict perm, const T* __restrict Dinv,
                                               const double* In, int64_t ldin, double* Out, int64_t ldout,
                                               int nrhs, uint64_t* msg, uint32_t seq, int64_t timeout,
                                               int64_t* err) {
-    constexpr int WPT = TMW / 256;  // message words per thread
+    constexpr int WPT = (2 * DB * NR + 255) / 256;  // message words per thread
+    constexpr int BPT = (DB * NR + 255) / 256;      // right-hand-side values per thread
     const int nblk = (int)((n + DB - 1) / DB);
     const int bw = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
     const int64_t r0 = (int64_t)bw * DB;
     const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
     const int64_t row = r0 + rr;
     __shared__ double Ds[DB][DB + 1];
-    __shared__ double Ys[2][DB][TRHS + 1];
-    __shared__ double Ps[4][DB][TRHS + 1];
+    __shared__ double Ys[2][DB][NR + 1];
+    __shared__ double Ps[4][DB][NR + 1];
+    double bin[BPT];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int idx = tid + 256 * i, k = idx & (DB - 1), s = idx >> 6;
+        const int64_t gr = r0 + k;
+        bin[i] = 0.0;
+        if (idx < DB * NR && gr < n && s < nrhs)
+            bin[i] = LOWER ? In[(int64_t)s * ldin + perm[gr]] : In[(int64_t)s * ldin + gr];
+    }
     for (int idx = tid; idx < DB * DB; idx += 256) {
         const int i = idx & (DB - 1), k = idx >> 6;
         Ds[i][k] = (double)Dinv[(int64_t)bw * DB * DB + k * DB + i];
     }
-    double acc[TRHS];
+    double acc[NR];
 #pragma unroll
-    for (int s = 0; s < TRHS; ++s) acc[s] = 0.0;
+    for (int s = 0; s < NR; ++s) acc[s] = 0.0;
     // word w of a block's message: value (row k, rhs s) half h at w = 2 (s DB + k) + h
     const int nj = LOWER ? bw : nblk - 1 - bw;  // blocks before this one in solve order
     auto jblk = [&](int t) { return LOWER ? t : nblk - 1 - t; };
@@ -627,7 +639,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
 #pragma unroll
         for (int i = 0; i < WPT; ++i) {
             const int w = tid + 256 * i;
-            x[i] = (w / (2 * DB) < nrhs) ? ld_relaxed_u64(m + w) : 0;
+            x[i] = (w < 2 * DB * NR && w / (2 * DB) < nrhs) ? ld_relaxed_u64(m + w) : 0;
         }
     };
     if (nj > 0) issue(0);
@@ -643,8 +655,10 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
         const uint64_t* m = msg + (size_t)j * TMW;
         uint32_t stale = 0;
 #pragma unroll
-        for (int i = 0; i < WPT; ++i)
-            if ((tid + 256 * i) / (2 * DB) < nrhs && (uint32_t)(x[i] >> 32) != seq) stale |= 1u << i;
+        for (int i = 0; i < WPT; ++i) {
+            const int w = tid + 256 * i;
+            if (w < 2 * DB * NR && w / (2 * DB) < nrhs && (uint32_t)(x[i] >> 32) != seq) stale |= 1u << i;
+        }
         if (stale) {
             const int64_t t0 = wall_clock64();
             while (stale) {
@@ -668,34 +682,33 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
             const uint32_t mine = (uint32_t)x[i];
             const uint32_t other = (uint32_t)__shfl_xor((int)mine, 1);
             const int k = (w >> 1) & (DB - 1), s = w / (2 * DB);
-            if (!(w & 1) && s < nrhs) Ys[buf][k][s] = __hiloint2double((int)other, (int)mine);
+            if (!(w & 1) && w < 2 * DB * NR && s < nrhs) Ys[buf][k][s] = __hiloint2double((int)other, (int)mine);
         }
         if (t + 1 < nj) issue(t + 1);  // the next block's words travel while this one is used
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 16; ++q)
 #pragma unroll
-            for (int s = 0; s < TRHS; ++s) acc[s] = fma(lv[q], Ys[buf][16 * g + q][s], acc[s]);
+            for (int s = 0; s < NR; ++s) acc[s] = fma(lv[q], Ys[buf][16 * g + q][s], acc[s]);
         // Ys is double-buffered: buffer buf is written again two iterations on, after the next
         // barrier
     }
 #pragma unroll
-    for (int s = 0; s < TRHS; ++s) Ps[g][rr][s] = acc[s];
+    for (int s = 0; s < NR; ++s) Ps[g][rr][s] = acc[s];
     __syncthreads();
     // right-hand side minus contributions, one (row, rhs) per thread-slot
-    double (*Yb)[TRHS + 1] = Ys[nj & 1];
-    for (int idx = tid; idx < DB * TRHS; idx += 256) {
-        const int i = idx & (DB - 1), s = idx >> 6;
-        const int64_t gr = r0 + i;
-        double b = 0.0;
-        if (gr < n && s < nrhs) b = LOWER ? In[(int64_t)s * ldin + perm[gr]] : In[(int64_t)s * ldin + gr];
-        Yb[i][s] = b - (Ps[0][i][s] + Ps[1][i][s] + Ps[2][i][s] + Ps[3][i][s]);
+    double (*Yb)[NR + 1] = Ys[nj & 1];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int idx = tid + 256 * i, k = idx & (DB - 1), s = idx >> 6;
+        if (idx < DB * NR) Yb[k][s] = bin[i] - (Ps[0][k][s] + Ps[1][k][s] + Ps[2][k][s] + Ps[3][k][s]);
     }
     __syncthreads();
     uint64_t* mine = msg + (size_t)bw * TMW;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int s = g + 4 * q;
+        if (s >= NR) continue;
         double y = 0.0;
 #pragma unroll 8
         for (int k = 0; k < DB; ++k) y = fma(Ds[rr][k], Yb[k][s], y);
