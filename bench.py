@@ -531,6 +531,20 @@ def main():
     ipiv = np.empty(n, dtype=np.int64)
     ctx.check(L.lsx_lu_get_factors(h, None, n, ipiv.ctypes.data), "lsx_lu_get_factors")
     log(f"backward error {be:.3e}")
+    # one right-hand side with the resident factors: forward and adjoint (A^T x = b) solves
+    db1 = dB[0].clone()
+    dx1 = torch.empty_like(db1)
+    solve_ms = {}
+    for name, fn in (("forward", L.lsx_lu_solve), ("adjoint", L.lsx_lu_solve_adjoint)):
+        fn(h, 1, db1.data_ptr(), n, dx1.data_ptr(), n)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ctx.check(fn(h, 1, db1.data_ptr(), n, dx1.data_ptr(), n), name)
+        torch.cuda.synchronize()
+        solve_ms[name] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+    log(f"one-RHS solves: forward {solve_ms['forward']} ms, adjoint {solve_ms['adjoint']} ms")
+    del db1, dx1
     # the same step with A and B in (pageable) host memory: the PCIe-inclusive rate (not `value`)
     Xh = np.empty_like(Gb)
     t0 = time.perf_counter()
@@ -582,6 +596,7 @@ def main():
         "solves_per_s": round(1e3 / ms, 3),
         "backward_error": be,
         "host_buffers_ms_per_step": round(host_ms, 3),
+        "one_rhs_solve_ms": solve_ms,
         "roofline": roofline,
         "kernel_breakdown": breakdown,
     }

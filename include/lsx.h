@@ -88,6 +88,12 @@ int lsx_lu_get_factors(lsx_lu* lu, double* F, int64_t ldf, int64_t* ipiv);
  * (ccall path src/openblas.jl:267-275). */
 int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
 
+/* Adjoint (= transpose, real data) solve with the same resident factors: X = A^{-T} B, i.e.
+ * LAPACK dgetrs with trans = 'T'.  Replaces the factorization reuse of solve!(cache; adjoint =
+ * true) -> _adjoint_factorization_solve -> F' \ b (src/common.jl:1012-1027,
+ * src/adjoint_factorization.jl:162-176).  FP64 handles only.  B and X may alias. */
+int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
+
 /* Per-kernel-class device timing of a handle (HIP events around each launch while enabled).
  * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update (128 x 128 tiles),
  * 3 permutation lists, 4 triangular solves, 5 refinement residual/update, 6 low-precision

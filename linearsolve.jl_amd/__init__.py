@@ -177,9 +177,49 @@ def init(prob, alg, *, alias_A=False, alias_b=False, abstol=None, reltol=None, m
     return cache
 
 
-def solve_(cache):
-    """solve!(cache) (src/common.jl:1012-1017) -> solve!(cache, cache.alg)."""
-    return cache.alg.solve_(cache)
+def solve_(cache, adjoint=False):
+    """solve!(cache) (src/common.jl:1012-1017) -> solve!(cache, cache.alg).  adjoint=True solves
+    adjoint(A) x = b (src/common.jl:1012-1027): a fresh cache is factorized first, then the
+    factorization is reused where the algorithm holds one (MI355XLUFactorization: getrs with
+    trans = 'T' on the device, src/adjoint_factorization.jl:162-176); otherwise the problem
+    with adjoint(A) is solved (Krylov methods with their preconditioners swapped and adjointed,
+    src/adjoint.jl:78-93; the others refactorized, src/adjoint_factorization.jl:245-264).
+    Writes cache.u; retcode Success."""
+    if not adjoint:
+        return cache.alg.solve_(cache)
+    alg = cache.alg
+    if cache.isfresh:
+        alg.solve_(cache)
+    if isinstance(alg, MI355XLUFactorization) and not cache.isfresh:
+        cache.cacheval.solve_adjoint(cache.b, cache.u)
+    else:
+        At = _adjoint_of(cache.A)
+        if isinstance(alg, MI355XSimpleGMRES):
+            # SimpleGMRES has a right slot: Pl' = adjoint(Pr), Pr' = adjoint(Pl) (diagonal: themselves)
+            sol = solve(LinearProblem(At, _copy(cache.b)), alg, abstol=cache.abstol, reltol=cache.reltol,
+                        maxiters=cache.maxiters, Pl=cache.Pr, Pr=cache.Pl, device=cache.ctx.device)
+        else:
+            sol = solve(LinearProblem(At, _copy(cache.b)), alg, abstol=cache.abstol, reltol=cache.reltol,
+                        device=cache.ctx.device)
+        if _is_torch(cache.u):
+            cache.u.copy_(sol.u if _is_torch(sol.u) else __import__("torch").from_numpy(np.asarray(sol.u)))
+        else:
+            cache.u[...] = _np(sol.u)
+    return LinearSolution(cache.u, None, ReturnCode.Success, 0, cache)
+
+
+def _adjoint_of(A):
+    """adjoint(A) for the matrix types the algorithms take (real data: the transpose)."""
+    if isinstance(A, CSRMatrix):
+        import scipy.sparse as sp
+        S = sp.csr_matrix((A.vals, A.colind, A.rowptr), shape=A.shape).T.tocsr()
+        S.sort_indices()
+        return CSRMatrix(S.indptr, S.indices, S.data, S.shape)
+    if isinstance(A, BlockDiagonal):
+        return BlockDiagonal([B.T for B in A.blocks])
+    if _is_torch(A):
+        return A.t().contiguous()
+    return np.ascontiguousarray(np.asarray(A).T)
 
 
 def solve(prob, alg, **kwargs):
@@ -225,6 +265,16 @@ class _LUHandle:
         self.ctx.check(_abi.lib().lsx_lu_get_factors(self.h, F.ctypes.data, self.n, ipiv.ctypes.data),
                        "lsx_lu_get_factors")
         return F, ipiv
+
+    def solve_adjoint(self, b, u):
+        keep, p = _colmajor(b)
+        nr = _ncols(b)
+        if _is_torch(u) and u.dim() == 2:
+            up, ldu = u.data_ptr(), self.n
+        else:
+            up, ldu = _ptr_out(u), self.n
+        self.ctx.check(_abi.lib().lsx_lu_solve_adjoint(self.h, nr, p, self.n, up, ldu), "lsx_lu_solve_adjoint")
+        return u
 
     def refine(self, b, u, tol, max_iters, mode):
         keep, p = _colmajor(b)

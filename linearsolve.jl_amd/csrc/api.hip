@@ -271,6 +271,31 @@ int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double*
     return LSX_OK;
 }
 
+int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx) {
+    if (!lu || !B || !X || nrhs < 0) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    if (!lu->factored) return fail(ctx, LSX_ESTATE, "lsx_lu_solve_adjoint: no factorization");
+    if (ldb < lu->n || ldx < lu->n) return fail(ctx, LSX_EINVAL, "lsx_lu_solve_adjoint: ldb/ldx < n");
+    if (lu->dtype != LSX_F64)
+        return fail(ctx, LSX_EINVAL, "lsx_lu_solve_adjoint: adjoint solves use FP64 handles");
+    if (nrhs == 0) return LSX_OK;
+    DeviceGuard g(ctx);
+    int rc = lu_ensure_work(lu, nrhs);
+    if (rc) return rc;
+    const int64_t n = lu->n;
+    LSX_HIP(ctx, hipMemcpy2DAsync(lu->wb, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    rc = lu64_getrs_t_dev(lu, nrhs, lu->wb, n, lu->wb, n);
+    if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpy2DAsync(X, sizeof(double) * ldx, lu->wb, sizeof(double) * n, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, lu->stream));
+    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t), hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    lu->prof.collect();
+    if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_solve_adjoint: a block wait exceeded its bound");
+    return LSX_OK;
+}
+
 int lsx_lu_profile(lsx_lu* lu, int32_t enable, lsx_profile* out) {
     if (!lu) return LSX_EINVAL;
     DeviceGuard g(lu->ctx);

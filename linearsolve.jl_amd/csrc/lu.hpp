@@ -71,6 +71,8 @@ void lu64_free(lsx_lu* lu);
 int lu64_getrf(lsx_lu* lu);
 // X = A \ B with device buffers (B n x nrhs, ldb; X n x nrhs, ldx; may alias).
 int lu64_getrs_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx);
+// X = A^{-T} B (device buffers, may alias).
+int lu64_getrs_t_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx);
 int lu_ensure_work(lsx_lu* lu, int64_t nrhs);
 // Mixed-precision (FP32 / BF16) factorization state, lu_mixed.hip.
 int mixed_alloc(lsx_lu* lu);

@@ -55,4 +55,8 @@ int lu64_getrs_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, doub
     return getrs_blocked<double>(lu, lu->F, lu->ld, lu->DL, lu->DU, nrhs, dB, ldb, dX, ldx);
 }
 
+int lu64_getrs_t_dev(lsx_lu* lu, int64_t nrhs, const double* dB, int64_t ldb, double* dX, int64_t ldx) {
+    return getrs_t_blocked<double>(lu, lu->F, lu->ld, lu->DL, lu->DU, nrhs, dB, ldb, dX, ldx);
+}
+
 }  // namespace lsx

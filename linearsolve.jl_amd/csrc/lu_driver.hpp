@@ -202,4 +202,31 @@ int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, 
     return LSX_OK;
 }
 
+// X = A^{-T} B with the resident factors (one right-hand side per launch pair; FP64 values).
+// The first solve writes lu->wy; X may alias B.
+template <typename T>
+int getrs_t_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, int64_t nrhs, const double* dB,
+                    int64_t ldb, double* dX, int64_t ldx) {
+    lsx_ctx* ctx = lu->ctx;
+    const int64_t n = lu->n;
+    const int nblk = (int)cdiv(n, DB);
+    if (lu->tseq > 0xF0000000u) {
+        LSX_HIP(ctx, hipMemsetAsync(lu->tmsg, 0, sizeof(uint64_t) * nblk * TMW, lu->stream));
+        lu->tseq = 1;
+    }
+    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
+    const int64_t timeout = ctx->wall_khz * 5000;
+    for (int64_t s = 0; s < nrhs; ++s) {
+        const uint32_t s1 = lu->tseq++, s2 = lu->tseq++;
+        const int pi = lu->prof.begin(lu->stream, PK_TRSV, 2.0 * n * n, sizeof(T) * (double)n * n);
+        hipLaunchKernelGGL((k_trsv_t<T, true>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DU,
+                           dB + s * ldb, lu->wy, lu->tmsg, s1, timeout, lu->dinfo + 1);
+        hipLaunchKernelGGL((k_trsv_t<T, false>), dim3(nblk), dim3(256), 0, lu->stream, F, ld, n, lu->perm, DL,
+                           (const double*)lu->wy, dX + s * ldx, lu->tmsg, s2, timeout, lu->dinfo + 1);
+        lu->prof.end(lu->stream, pi);
+    }
+    LSX_HIP(ctx, hipGetLastError());
+    return LSX_OK;
+}
+
 }  // namespace lsx

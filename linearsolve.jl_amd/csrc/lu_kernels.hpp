@@ -720,4 +720,98 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
     }
 }
 
+// Transposed triangular solves (getrs with trans = 'T', A^T x = b with P A = L U:
+// U^T w = b, L^T v = w, x = P^T v), one workgroup per DB-row block, one right-hand side.
+//   FIRST:  Out = U^{-T} In         (U^T is lower triangular: blocks in forward order)
+//   !FIRST: Out[perm[i]] = (L^{-T} In)[i]   (L^T unit upper: blocks in backward order)
+// The tile of block row b and block column j of the transposed factor is the (j, b) tile of F
+// read by columns: lane r of wave g reads F[(r0 + 16 g + q) ld + j DB + r] (coalesced), keeps
+// its own partial sums of the 16 output rows 16 g + q over all earlier blocks, and the sums are
+// reduced across the wave once at the end.  The diagonal block's inverse is the transpose of the
+// stored one.  Block solutions travel as in k_trsv (two 64-bit words per value, 32-bit halves
+// each with the launch's sequence number).
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t ld, int64_t n,
+                                                const int* __restrict perm, const T* __restrict Dinv,
+                                                const double* In, double* Out, uint64_t* msg, uint32_t seq,
+                                                int64_t timeout, int64_t* err) {
+    const int nblk = (int)((n + DB - 1) / DB);
+    const int bw = FIRST ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+    const int64_t r0 = (int64_t)bw * DB;
+    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+    __shared__ double Ds[DB][DB + 1];  // Ds[i][k] = Dinv^T(i, k)
+    __shared__ double Wb[2][DB];       // solution of the block being consumed (double-buffered)
+    __shared__ double Red[DB];
+    __shared__ double Yb[DB];
+    double bin = 0.0;
+    if (tid < DB && r0 + tid < n) bin = In[r0 + tid];
+    for (int idx = tid; idx < DB * DB; idx += 256) {
+        const int i = idx >> 6, k = idx & (DB - 1);
+        Ds[i][k] = (double)Dinv[(int64_t)bw * DB * DB + (int64_t)i * DB + k];
+    }
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    const int nj = FIRST ? bw : nblk - 1 - bw;
+    auto jblk = [&](int t) { return FIRST ? t : nblk - 1 - t; };
+    uint64_t x = 0;
+    auto issue = [&](int t) {
+        if (tid < 2 * DB) x = ld_relaxed_u64(msg + (size_t)jblk(t) * TMW + tid);
+    };
+    if (nj > 0) issue(0);
+    for (int t = 0; t < nj; ++t) {
+        const int j = jblk(t), buf = t & 1;
+        const int64_t srow = (int64_t)j * DB + r;
+        double lv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t ocol = r0 + 16 * g + q;
+            lv[q] = (ocol < n && srow < n) ? (double)F[ocol * ld + srow] : 0.0;
+        }
+        if (tid < 2 * DB && (uint32_t)(x >> 32) != seq) {
+            const uint64_t* m = msg + (size_t)j * TMW + tid;
+            const int64_t t0 = wall_clock64();
+            while (true) {
+                __builtin_amdgcn_s_sleep(1);
+                x = ld_relaxed_u64(m);
+                if ((uint32_t)(x >> 32) == seq) break;
+                if (wall_clock64() - t0 > timeout) {
+                    atomicOr((unsigned long long*)err, 2ull);
+                    break;
+                }
+            }
+        }
+        {
+            const uint32_t mine = (uint32_t)x;
+            const uint32_t other = (uint32_t)__shfl_xor((int)mine, 1);
+            if (tid < 2 * DB && !(tid & 1)) Wb[buf][tid >> 1] = __hiloint2double((int)other, (int)mine);
+        }
+        if (t + 1 < nj) issue(t + 1);
+        __syncthreads();
+        const double wr = Wb[buf][r];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = fma(lv[q], wr, acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (r == 0) Red[16 * g + q] = v;
+    }
+    __syncthreads();
+    if (tid < DB) Yb[tid] = bin - Red[tid];
+    __syncthreads();
+    if (tid < DB) {
+        double y = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < DB; ++k) y = fma(Ds[tid][k], Yb[k], y);
+        const int64_t row = r0 + tid;
+        if (row < n) Out[FIRST ? row : (int64_t)perm[row]] = y;
+        uint64_t* mine = msg + (size_t)bw * TMW;
+        st_relaxed_u64(mine + 2 * tid, ll_pack((uint32_t)__double2loint(y), seq));
+        st_relaxed_u64(mine + 2 * tid + 1, ll_pack((uint32_t)__double2hiint(y), seq));
+    }
+}
+
 }  // namespace lsx

@@ -212,3 +212,69 @@ def test_config2_full_size_pivots_bit_exact(lsx, n):
     lu, piv, il = lapack.dgetrf(A, overwrite_a=True)
     mism = np.nonzero(ipiv != piv + 1)[0]
     assert mism.size == 0, f"pivot mismatch at columns {mism[:10]}"
+
+
+# ---- adjoint solves: solve!(cache; adjoint = true) reuses the factorization (getrs 'T')
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 300, 1000, 2049])
+def test_adjoint_solve_matches_lapack_getrs_t(lsx, n):
+    # A^T x = b with the resident factors: backward error of the transposed system <= 1e-13, and
+    # the solution equal (to rounding) to LAPACK dgetrs with trans = 'T' on the same factors
+    print(f"adjoint n={n}", flush=True)
+    rng = np.random.default_rng(3000 + n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 3))
+    h, info, F, ipiv = factor(lsx, A)
+    assert info == 0
+    X = np.empty((n, 3), order="F")
+    h.solve_adjoint(np.asfortranarray(B), X)
+    assert backward_error(A.T, X, B) <= 1e-13
+    Xl, linfo = lapack.dgetrs(F, ipiv - 1, B, trans=1)
+    assert linfo == 0
+    np.testing.assert_allclose(X, Xl, rtol=0, atol=1e-10 * np.abs(Xl).max())
+    # one right-hand side (the vector case)
+    x = np.empty(n)
+    h.solve_adjoint(B[:, 0].copy(), x)
+    assert backward_error(A.T, x, B[:, 0]) <= 1e-13
+
+
+def test_reference_adjoint_cache_cases(lsx):
+    # test/Core/adjoint.jl:446-498 (LUFactorization): a primal solve, then cache.b = c and an
+    # adjoint solve from the same factorization; both directions from one cache; a fresh cache
+    # factorizes first
+    rng = np.random.default_rng(92)
+    m = 60
+    A = rng.random((m, m)) + m * np.eye(m)
+    b = rng.random(m)
+    c = rng.random(m)
+    cache = lsx.init(lsx.LinearProblem(A, b), lsx.MI355XLUFactorization())
+    np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A, b), rtol=1e-12)
+    cache.b = c
+    sol = lsx.solve_(cache, adjoint=True)
+    assert sol.retcode == lsx.ReturnCode.Success
+    np.testing.assert_allclose(sol.u, np.linalg.solve(A.T, c), rtol=1e-12)
+    np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A, c), rtol=1e-12)  # and back again
+    A2 = rng.random((m, m)) + m * np.eye(m)
+    cache2 = lsx.init(lsx.LinearProblem(A2, b), lsx.MI355XLUFactorization())
+    np.testing.assert_allclose(lsx.solve_(cache2, adjoint=True).u, np.linalg.solve(A2.T, b), rtol=1e-12)
+
+
+def test_adjoint_fallbacks(lsx):
+    # algorithms without factorization reuse solve the problem with adjoint(A): the mixed LU
+    # refactorizes A^T, SimpleGMRES solves with A^T and its preconditioners swapped
+    import scipy.sparse as sp
+    from linearsolve_jl_amd import problems as P
+    rng = np.random.default_rng(5)
+    n = 200
+    A = rng.standard_normal((n, n)) + 0.5 * n ** 0.5 * np.eye(n)
+    b = rng.standard_normal(n)
+    cache = lsx.init(lsx.LinearProblem(A, b), lsx.MI355XMixedLUFactorization(precision="fp32"))
+    x = lsx.solve_(cache, adjoint=True).u
+    assert backward_error(A.T, x, b) <= 1e-13
+    rp, ci, v = P.convection_diffusion(30, 4.0)
+    S = sp.csr_matrix((v, ci, rp), shape=(900, 900))
+    bb = rng.standard_normal(900)
+    cache = lsx.init(lsx.LinearProblem(lsx.CSRMatrix(rp, ci, v, (900, 900)), bb), lsx.MI355XSimpleGMRES(memory=30),
+                     Pl=np.full(900, 4.0), reltol=1e-10, maxiters=5000)
+    x = lsx.solve_(cache, adjoint=True).u
+    assert np.linalg.norm(S.T @ x - bb) <= 1e-8 * np.linalg.norm(bb)
