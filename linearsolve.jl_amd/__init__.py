@@ -27,7 +27,7 @@ __all__ = [
     "ReturnCode", "LinearProblem", "LinearCache", "LinearSolution", "init", "solve", "solve_",
     "MI355XLUFactorization", "MI355XMixedLUFactorization", "MI355XBatchedLUFactorization",
     "MI355XSimpleGMRES", "BlockDiagonal", "CSRMatrix", "LSXError", "Context", "Server",
-    "default_tol",
+    "default_tol", "default_algorithm", "OperatorAssumptions",
 ]
 
 
@@ -153,11 +153,54 @@ class LinearCache:
         object.__setattr__(self, "isfresh", True)
 
 
-def init(prob, alg, *, alias_A=False, alias_b=False, abstol=None, reltol=None, maxiters=None,
-         Pl=None, Pr=None, verbose=False, device=0):
+CONDITIONS = ("WellConditioned", "IllConditioned", "VeryIllConditioned", "SuperIllConditioned")
+
+
+class OperatorAssumptions:
+    """OperatorAssumptions (src/common.jl:184-198): issq (None = from A's shape) and the
+    conditioning the caller vouches for, IllConditioned by default."""
+
+    def __init__(self, issq=None, condition="IllConditioned"):
+        if condition not in CONDITIONS:
+            raise ValueError(f"condition must be one of {CONDITIONS}")
+        self.issq = issq
+        self.condition = condition
+
+
+def default_algorithm(A, b, assumptions=None):
+    """The algorithm solve(prob) picks without one: a device slot of defaultalg mapped onto this
+    build's algorithms, with the conditioning thresholds of the reference's dense route
+    (src/default.jl:411-477: LU for Well- and IllConditioned, QR for VeryIllConditioned, SVD for
+    SuperIllConditioned) instead of its GPU route's QR for the default IllConditioned
+    (:277-306), and a GMRES Krylov method for sparse operators (:327-343):
+      dense square (numpy or torch)  -> MI355XLUFactorization
+      CSRMatrix                      -> MI355XSimpleGMRES (the device GMRES)
+      BlockDiagonal                  -> MI355XBatchedLUFactorization (blockwise LU)
+    QR, SVD and the non-square Krylov methods are not built here: those choices raise."""
+    assumptions = assumptions or OperatorAssumptions()
+    if isinstance(A, BlockDiagonal):
+        return MI355XBatchedLUFactorization()
+    if isinstance(A, CSRMatrix):
+        if A.shape[0] != A.shape[1]:
+            raise NotImplementedError("non-square sparse systems (LSMR / CRAIGMR) are not built")
+        return MI355XSimpleGMRES()
+    shape = tuple(A.shape)
+    issq = (shape[0] == shape[1]) if assumptions.issq is None else assumptions.issq
+    if not issq or len(shape) != 2 or shape[0] != shape[1]:
+        raise NotImplementedError("non-square dense systems take QR in the reference; QR is not built")
+    if assumptions.condition in ("VeryIllConditioned", "SuperIllConditioned"):
+        raise NotImplementedError(f"{assumptions.condition} takes QR / SVD in the reference; they are not built")
+    return MI355XLUFactorization()
+
+
+def init(prob, alg=None, *, alias_A=False, alias_b=False, abstol=None, reltol=None, maxiters=None,
+         Pl=None, Pr=None, verbose=False, device=0, assumptions=None):
     """SciMLBase.init -> __init (src/common.jl:758-931): copies A and b unless aliased (dense and
     Krylov algorithms default to no aliasing, :525-544; simplegmres.jl:185-186), zero u, default
-    tolerances sqrt(eps) (:411) and maxiters = length(b) (:764)."""
+    tolerances sqrt(eps) (:411) and maxiters = length(b) (:764).  Without alg, the default
+    algorithm for A (default_algorithm)."""
+    if alg is None:
+        alg = default_algorithm(prob.A, prob.b, assumptions)
     A = prob.A if alias_A else (_copy(prob.A) if not isinstance(prob.A, (CSRMatrix, BlockDiagonal)) else prob.A)
     b = prob.b if alias_b else _copy(prob.b)
     if isinstance(alg, MI355XSimpleGMRES) and len(b.shape) == 2:
@@ -222,8 +265,9 @@ def _adjoint_of(A):
     return np.ascontiguousarray(np.asarray(A).T)
 
 
-def solve(prob, alg, **kwargs):
-    """solve(prob, alg) = solve!(init(prob, alg)) (src/common.jl:988-993)."""
+def solve(prob, alg=None, **kwargs):
+    """solve(prob, alg) = solve!(init(prob, alg)) (src/common.jl:988-993); solve(prob) picks
+    default_algorithm."""
     return solve_(init(prob, alg, **kwargs))
 
 

@@ -86,3 +86,29 @@ def test_convection_diffusion_generator():
     S = sp.csr_matrix((v, ci, rp), shape=(m * m, m * m)).toarray()
     h = 1.0 / (m + 1)
     assert np.isclose(S[10, 11], -1 + 8.0 * h / 2) and np.isclose(S[11, 10], -1 - 8.0 * h / 2)
+
+
+def test_default_algorithm_routes():
+    # solve(prob) without an algorithm: dense square -> LU, CSR -> SimpleGMRES, BlockDiagonal ->
+    # batched LU; the QR / SVD choices of the reference (non-square, VeryIll / SuperIllConditioned)
+    # are not built and raise.  Host logic only (no device calls).
+    import numpy as np
+    import pytest
+    from conftest import load_package
+    lsx = load_package()
+    A = np.eye(4)
+    b = np.ones(4)
+    assert isinstance(lsx.default_algorithm(A, b), lsx.MI355XLUFactorization)
+    assert isinstance(lsx.default_algorithm(A, b, lsx.OperatorAssumptions(condition="WellConditioned")),
+                      lsx.MI355XLUFactorization)
+    S = lsx.CSRMatrix(np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32), np.ones(4), (4, 4))
+    assert isinstance(lsx.default_algorithm(S, b), lsx.MI355XSimpleGMRES)
+    assert isinstance(lsx.default_algorithm(lsx.BlockDiagonal([np.eye(2), np.eye(2)]), b),
+                      lsx.MI355XBatchedLUFactorization)
+    for cond in ("VeryIllConditioned", "SuperIllConditioned"):
+        with pytest.raises(NotImplementedError):
+            lsx.default_algorithm(A, b, lsx.OperatorAssumptions(condition=cond))
+    with pytest.raises(NotImplementedError):
+        lsx.default_algorithm(np.ones((4, 3)), b)
+    with pytest.raises(ValueError):
+        lsx.OperatorAssumptions(condition="Unknown")

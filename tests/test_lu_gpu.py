@@ -278,3 +278,33 @@ def test_adjoint_fallbacks(lsx):
                      Pl=np.full(900, 4.0), reltol=1e-10, maxiters=5000)
     x = lsx.solve_(cache, adjoint=True).u
     assert np.linalg.norm(S.T @ x - bb) <= 1e-8 * np.linalg.norm(bb)
+
+
+def test_default_solver_routes_to_device_algorithms(lsx):
+    # solve(prob) with no algorithm: dense -> MI355XLUFactorization, CSR -> MI355XSimpleGMRES,
+    # BlockDiagonal -> MI355XBatchedLUFactorization, each answering to its own bar
+    import scipy.sparse as sp
+    from linearsolve_jl_amd import problems as P
+    rng = np.random.default_rng(77)
+    A = rng.standard_normal((300, 300))
+    b = rng.standard_normal(300)
+    sol = lsx.solve(lsx.LinearProblem(A, b))
+    assert isinstance(sol.cache.alg, lsx.MI355XLUFactorization)
+    assert sol.retcode == lsx.ReturnCode.Success and backward_error(A, sol.u, b) <= 1e-13
+    rp, ci, v = P.convection_diffusion(20, 2.0)
+    S = lsx.CSRMatrix(rp, ci, v, (400, 400))
+    bs = rng.standard_normal(400)
+    sol = lsx.solve(lsx.LinearProblem(S, bs), Pl=np.full(400, 4.0), reltol=1e-10)
+    assert isinstance(sol.cache.alg, lsx.MI355XSimpleGMRES) and sol.retcode == lsx.ReturnCode.Success
+    Ssp = sp.csr_matrix((v, ci, rp), shape=(400, 400))
+    assert np.linalg.norm(Ssp @ sol.u - bs) <= 1e-8 * np.linalg.norm(bs)
+    blocks = [P.dense_block(n, seed=n)[0] for n in (5, 17, 40)]
+    Bd = lsx.BlockDiagonal(blocks)
+    bb = rng.standard_normal(62)
+    sol = lsx.solve(lsx.LinearProblem(Bd, bb))
+    assert isinstance(sol.cache.alg, lsx.MI355XBatchedLUFactorization) and sol.retcode == lsx.ReturnCode.Success
+    off = 0
+    for B in blocks:
+        n = B.shape[0]
+        assert backward_error(B, sol.u[off:off + n], bb[off:off + n]) <= 1e-13
+        off += n
