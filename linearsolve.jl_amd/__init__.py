@@ -27,7 +27,7 @@ __all__ = [
     "ReturnCode", "LinearProblem", "LinearCache", "LinearSolution", "init", "solve", "solve_",
     "MI355XLUFactorization", "MI355XMixedLUFactorization", "MI355XBatchedLUFactorization",
     "MI355XSimpleGMRES", "BlockDiagonal", "CSRMatrix", "LSXError", "Context", "Server",
-    "default_tol", "default_algorithm", "OperatorAssumptions",
+    "default_tol", "default_algorithm", "OperatorAssumptions", "resize_",
 ]
 
 
@@ -251,6 +251,14 @@ def solve_(cache, adjoint=False):
     return LinearSolution(cache.u, None, ReturnCode.Success, 0, cache)
 
 
+def resize_(cache, i):
+    """resize!(cache, i) (src/common.jl:362-366): the cache is marked fresh; the device handle is
+    kept and replaced on the next solve only if the order changed (a handle of the same order
+    and type is reused: refactorization allocates nothing on the device)."""
+    object.__setattr__(cache, "isfresh", True)
+    return cache
+
+
 def _adjoint_of(A):
     """adjoint(A) for the matrix types the algorithms take (real data: the transpose)."""
     if isinstance(A, CSRMatrix):
@@ -293,14 +301,26 @@ class _LUHandle:
         keep, p = _colmajor(A)
         return self.ctx.check(_abi.lib().lsx_lu_factor(self.h, p, self.n), "lsx_lu_factor")
 
+    def _out(self, u):
+        """(target, pointer, ld, copy_back) for writing a column-major result into u."""
+        if _is_torch(u):
+            if u.dim() == 2 and not (u.stride(0) == 1 and u.stride(1) == self.n):
+                import torch
+                t = torch.empty((u.shape[1], u.shape[0]), dtype=torch.float64, device=u.device).t()
+                return t, t.data_ptr(), self.n, True
+            return u, u.data_ptr(), self.n, False
+        if u.ndim == 2 and not u.flags.f_contiguous:
+            t = np.empty(u.shape, dtype=np.float64, order="F")
+            return t, t.ctypes.data, self.n, True
+        return u, _ptr_out(u), self.n, False
+
     def solve(self, b, u):
         keep, p = _colmajor(b)
         nr = _ncols(b)
-        if _is_torch(u) and u.dim() == 2:
-            up, ldu = u.data_ptr(), self.n  # u is a column-major view
-        else:
-            up, ldu = _ptr_out(u), self.n
+        t, up, ldu, back = self._out(u)
         self.ctx.check(_abi.lib().lsx_lu_solve(self.h, nr, p, self.n, up, ldu), "lsx_lu_solve")
+        if back:
+            u[...] = t
         return u
 
     def factors(self):
@@ -313,11 +333,10 @@ class _LUHandle:
     def solve_adjoint(self, b, u):
         keep, p = _colmajor(b)
         nr = _ncols(b)
-        if _is_torch(u) and u.dim() == 2:
-            up, ldu = u.data_ptr(), self.n
-        else:
-            up, ldu = _ptr_out(u), self.n
+        t, up, ldu, back = self._out(u)
         self.ctx.check(_abi.lib().lsx_lu_solve_adjoint(self.h, nr, p, self.n, up, ldu), "lsx_lu_solve_adjoint")
+        if back:
+            u[...] = t
         return u
 
     def refine(self, b, u, tol, max_iters, mode):

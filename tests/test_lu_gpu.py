@@ -308,3 +308,32 @@ def test_default_solver_routes_to_device_algorithms(lsx):
         n = B.shape[0]
         assert backward_error(B, sol.u[off:off + n], bb[off:off + n]) <= 1e-13
         off += n
+
+
+def test_refactorization_reuses_device_handle(lsx):
+    # test/Core/lu_refactorization.jl:28-81: repeated refactorize + solve cycles on one cache are
+    # correct, and after warm-up a refactorization reuses the device factor / pivot buffers (the
+    # same handle) instead of allocating; resize_ marks the cache fresh and keeps the handle
+    rng = np.random.default_rng(51)
+    n = 51
+    A1 = rng.random((n, n)) + n * np.eye(n)
+    A2 = rng.random((n, n)) + n * np.eye(n)
+    b = rng.random(n)
+    cache = lsx.init(lsx.LinearProblem(A1.copy(), b.copy()), lsx.MI355XLUFactorization(), alias_A=True)
+    np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A1, b), rtol=1e-12)
+    handle = cache.cacheval.h.value
+    for Ak in (A2, A1, A2):
+        cache.A = Ak.copy()
+        sol = lsx.solve_(cache)
+        assert sol.retcode == lsx.ReturnCode.Success
+        np.testing.assert_allclose(sol.u, np.linalg.solve(Ak, b), rtol=1e-12)
+        assert cache.cacheval.h.value == handle
+    lsx.resize_(cache, n)
+    assert cache.isfresh
+    np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A2, b), rtol=1e-12)
+    assert cache.cacheval.h.value == handle
+    # matrix right-hand side (test/Core/batch.jl:9-49): every column solved
+    B = rng.random((n, 4))
+    cache.b = B
+    cache.u = np.zeros((n, 4))
+    np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A2, B), rtol=1e-12)
