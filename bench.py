@@ -364,7 +364,69 @@ def run_gmres(lsx, _abi, ctx, L, m, pe, memory, maxiters, cpu_m):
             f"{ent['achieved_GBs_minimum']} GB/s (fused minimum bytes)" + (f"; CPU oracle {ent['cpu_oracle']['ms']:.0f} ms"
                                                          if "cpu_oracle" in ent else ""))
         del db, dx, dpl
+    out["batched"] = run_gmres_batched(lsx, _abi, ctx, L)
     return dict(memory=memory, maxiters=maxiters, pe=pe, precond="Jacobi (diagonal 4)", runs=out)
+
+
+def run_gmres_batched(lsx, _abi, ctx, L, nblocks=4096, side=16, loop_sample=64):
+    """SimpleGMRES(blocksize) on a block-diagonal matrix of `nblocks` convection-diffusion
+    blocks (side x side grids, Pe cycling 0..9), one launch, against a loop of per-block
+    SimpleGMRES solves (a sample of the same blocks, handles made beforehand)."""
+    import torch
+    import scipy.sparse as sp
+    P = lsx.problems
+    bs = side * side
+    mats = [P.convection_diffusion(side, float(k % 10)) for k in range(10)]
+    blocks = [sp.csr_matrix((v, ci, rp), shape=(bs, bs)) for rp, ci, v in mats]
+    S = sp.block_diag([blocks[k % 10] for k in range(nblocks)], format="csr")
+    S.sort_indices()
+    N = S.shape[0]
+    b = P.rng_for(13, nblocks).standard_normal(N)
+    tol = float(np.sqrt(np.finfo(np.float64).eps))
+    # the int32 copies must stay referenced while the library reads them
+    rp32, ci32, vals = S.indptr.astype(np.int32), S.indices.astype(np.int32), np.ascontiguousarray(S.data)
+    A = ctypes.c_void_p()
+    ctx.check(L.lsx_csr_create(ctx.handle, N, S.nnz, rp32.ctypes.data, ci32.ctypes.data, vals.ctypes.data,
+                               ctypes.byref(A)), "lsx_csr_create")
+    db = torch.from_numpy(b).cuda()
+    dx = torch.empty_like(db)
+    dpl = torch.full((N,), 4.0, dtype=torch.float64, device="cuda")
+    st = _abi.GmresStats()
+
+    def batched():
+        ctx.check(L.lsx_gmres_solve_batched(A, bs, db.data_ptr(), dx.data_ptr(), 20, 1, tol, 1e-8, 1000,
+                                            dpl.data_ptr(), None, ctypes.byref(st)), "lsx_gmres_solve_batched")
+
+    tb, _ = timed(batched, 3)
+    L.lsx_csr_destroy(A)
+    # the loop: one handle per distinct block, solves timed
+    hs = []
+    for B in blocks:
+        h = ctypes.c_void_p()
+        brp, bci, bv = B.indptr.astype(np.int32), B.indices.astype(np.int32), np.ascontiguousarray(B.data)
+        ctx.check(L.lsx_csr_create(ctx.handle, bs, B.nnz, brp.ctypes.data, bci.ctypes.data, bv.ctypes.data,
+                                   ctypes.byref(h)), "lsx_csr_create")
+        hs.append(h)
+    eps = tol + 1e-8 * float(np.linalg.norm(b / 4.0))
+    st1 = _abi.GmresStats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(loop_sample):
+        sl = slice(k * bs, (k + 1) * bs)
+        ctx.check(L.lsx_gmres_solve(hs[k % 10], db[sl].data_ptr(), dx[sl].data_ptr(), 20, 1, eps, 0.0, 1000,
+                                    dpl[sl].data_ptr(), None, ctypes.byref(st1)), "lsx_gmres_solve")
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) / loop_sample
+    for h in hs:
+        L.lsx_csr_destroy(h)
+    res = dict(blocks=nblocks, blocksize=bs, N=N, ms=round(tb * 1e3, 3), solves_per_s=round(nblocks / tb, 1),
+               iterations_slowest_block=int(st.iters), retcode=int(st.retcode),
+               per_block_loop=dict(solves_per_s=round(1.0 / tl, 1), sample=loop_sample),
+               speedup_vs_loop=round(nblocks / tb * tl, 2))
+    log(f"gmres batched: {nblocks} blocks of {bs} in {tb * 1e3:.1f} ms ({res['solves_per_s']} solves/s), "
+        f"loop {res['per_block_loop']['solves_per_s']} solves/s -> {res['speedup_vs_loop']}x")
+    del db, dx, dpl
+    return res
 
 
 def _submit_synth(srv, req, gmres_maxiters):

@@ -180,6 +180,17 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
                     double abstol, double reltol, int64_t maxiters, const double* pl_diag,
                     const double* pr_diag, lsx_gmres_stats* stats);
 
+/* SimpleGMRES with blocksize > 0 (the batched layout, src/simplegmres.jl:190-373): A is block
+ * diagonal with N / blocksize uniform blocks, and every block runs its own GMRES.  Each block
+ * stops when its residual estimate reaches eps = abstol + reltol ||Pl \ b||_2 (the norm of the
+ * whole vector), the reference's test on its slowest block; stats carry the slowest block's
+ * iterations and rNorm, Infeasible if any block is, else Success if all are solved, else
+ * MaxIters.  The blocks run as one multi-solve launch.  Returns LSX_EINVAL when A has entries
+ * outside the diagonal blocks or blocksize exceeds 8192. */
+int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, double* x, int32_t memory,
+                            int32_t restart, double abstol, double reltol, int64_t maxiters,
+                            const double* pl_diag, const double* pr_diag, lsx_gmres_stats* stats);
+
 /* ---------------------------------------------------------------- serving ---------------- */
 typedef struct {
     int32_t kind;          /* 0 = dense LU solve, 1 = SimpleGMRES */

@@ -649,12 +649,13 @@ class MI355XSimpleGMRES:
     needs_square_A = True
 
     def __init__(self, restart=True, memory=20, blocksize=0, warm_start=False):
-        if blocksize != 0:
-            raise ValueError("the uniform block-diagonal layout is not implemented")
+        if blocksize < 0:
+            raise ValueError("blocksize must be >= 0")
         if warm_start:
             raise ValueError("warm_start is not supported (it does not seed from u0 in the reference)")
         self.restart = restart
         self.memory = memory
+        self.blocksize = blocksize  # > 0: block-diagonal A, one GMRES per block (the batched layout)
 
     def init_cacheval(self, cache):
         return None
@@ -672,10 +673,18 @@ class MI355XSimpleGMRES:
         pr = _diag_of(cache.Pr, n)
         kb, pb = _colmajor(cache.b)
         st = _abi.GmresStats()
-        cache.ctx.check(_abi.lib().lsx_gmres_solve(
-            cache.cacheval.h, pb, _ptr_out(cache.u), self.memory, int(bool(self.restart)), cache.abstol,
-            cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
-            pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve")
+        if self.blocksize > 0:
+            if n % self.blocksize:
+                raise ValueError("the blocksize must divide the size of the matrix")  # simplegmres.jl:422
+            cache.ctx.check(_abi.lib().lsx_gmres_solve_batched(
+                cache.cacheval.h, self.blocksize, pb, _ptr_out(cache.u), self.memory, int(bool(self.restart)),
+                cache.abstol, cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
+                pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve_batched")
+        else:
+            cache.ctx.check(_abi.lib().lsx_gmres_solve(
+                cache.cacheval.h, pb, _ptr_out(cache.u), self.memory, int(bool(self.restart)), cache.abstol,
+                cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
+                pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve")
         return LinearSolution(cache.u, st.resid, ReturnCode(st.retcode), int(st.iters), cache)
 
 

@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <new>
+#include <vector>
+#include <cmath>
 
 #include "gmres.hpp"
 
@@ -556,12 +558,57 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
 
 }  // namespace lsx
 
+namespace lsx {
+// CSR structure check: rowptr[0] = 0, rowptr[N] = nnz, rows non-decreasing within [0, nnz], column
+// indices in [0, N); *bad is set otherwise.  A row's entries are read only once its bounds passed.
+__global__ void k_csr_check(int64_t N, int64_t nnz, const int32_t* __restrict rowptr, const int32_t* __restrict col,
+                            int* bad) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k0 = rowptr[r], k1 = rowptr[r + 1];
+        if ((r == 0 && k0 != 0) || (r == N - 1 && k1 != nnz) || k0 < 0 || k1 < k0 || k1 > nnz) {
+            *bad = 1;
+            continue;
+        }
+        for (int64_t k = k0; k < k1; ++k)
+            if (col[k] < 0 || col[k] >= N) *bad = 1;
+    }
+}
+
+int csr_validate(hipStream_t st, int64_t N, int64_t nnz, const int32_t* rowptr, const int32_t* col, int* dflag) {
+    if (hipMemsetAsync(dflag, 0, sizeof(int), st) != hipSuccess) return LSX_EHIP;
+    hipLaunchKernelGGL(k_csr_check, dim3((unsigned)std::min<int64_t>(cdiv(N, 256), 4096)), dim3(256), 0, st, N, nnz,
+                       rowptr, col, dflag);
+    return hipGetLastError() == hipSuccess ? LSX_OK : LSX_EHIP;
+}
+}  // namespace lsx
+
+// Column indices local to each row's block of `bs` rows (block-diagonal matrices); *bad is set
+// when an entry lies outside its row's diagonal block.
+__global__ void k_local_cols(int64_t N, int64_t bs, const int32_t* __restrict rowptr, const int32_t* __restrict col,
+                             int32_t* __restrict cl, int* bad) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b0 = (r / bs) * bs;
+        for (int32_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const int64_t c = col[k];
+            if (c < b0 || c >= b0 + bs) *bad = 1;
+            cl[k] = (int32_t)(c - b0);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // C-ABI
 using namespace lsx;
 
 static void csr_free(lsx_csr* A) {
     if (A->stream) (void)hipStreamSynchronize(A->stream);
+    (void)hipFree(A->col_local);
+    (void)hipFree(A->bred);
+    (void)hipFree(A->bitems);
+    (void)hipFree(A->bfirst);
+    (void)hipFree(A->bout);
+    (void)hipFree(A->bticket);
+    (void)hipFree(A->bflag);
     (void)hipFree(A->rowptr);
     (void)hipFree(A->col);
     (void)hipFree(A->val);
@@ -604,13 +651,22 @@ int lsx_csr_create(lsx_ctx* ctx, int64_t N, int64_t nnz, const int32_t* rowptr, 
         hipMalloc(&A->out, sizeof(GmresOut)) != hipSuccess ||
         hipHostMalloc(&A->hout, sizeof(GmresOut), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&A->db, sizeof(double) * N) != hipSuccess || hipMalloc(&A->dx, sizeof(double) * N) != hipSuccess ||
-        hipMalloc(&A->dpl, sizeof(double) * N) != hipSuccess || hipMalloc(&A->dpr, sizeof(double) * N) != hipSuccess)
+        hipMalloc(&A->dpl, sizeof(double) * N) != hipSuccess || hipMalloc(&A->dpr, sizeof(double) * N) != hipSuccess ||
+        hipMalloc(&A->bflag, sizeof(int)) != hipSuccess)
         return bad(fail(ctx, LSX_ENOMEM, "lsx_csr_create: device allocation failed (N = %lld)", (long long)N));
     if (hipMemcpyAsync(A->rowptr, rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDefault, A->stream) != hipSuccess ||
         (nnz > 0 && hipMemcpyAsync(A->col, colind, sizeof(int32_t) * nnz, hipMemcpyDefault, A->stream) != hipSuccess) ||
         (nnz > 0 && hipMemcpyAsync(A->val, vals, sizeof(double) * nnz, hipMemcpyDefault, A->stream) != hipSuccess) ||
         hipStreamSynchronize(A->stream) != hipSuccess)
         return bad(fail(ctx, LSX_EHIP, "lsx_csr_create: upload failed"));
+    int badcsr = 0;
+    if (csr_validate(A->stream, N, nnz, A->rowptr, A->col, A->bflag) != LSX_OK ||
+        hipMemcpyAsync(&badcsr, A->bflag, sizeof(int), hipMemcpyDeviceToHost, A->stream) != hipSuccess ||
+        hipStreamSynchronize(A->stream) != hipSuccess)
+        return bad(fail(ctx, LSX_EHIP, "lsx_csr_create: structure check failed to run"));
+    if (badcsr)
+        return bad(fail(ctx, LSX_EINVAL, "lsx_csr_create: not a CSR matrix of order %lld with %lld entries (row pointers "
+                        "or column indices out of range)", (long long)N, (long long)nnz));
     *out = A;
     return LSX_OK;
 }
@@ -658,6 +714,149 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
         stats->pad = 0;
         stats->iters = A->hout->iters;
         stats->resid = A->hout->resid;
+    }
+    return LSX_OK;
+}
+
+int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, double* x, int32_t memory,
+                            int32_t restart, double abstol, double reltol, int64_t maxiters, const double* pl_diag,
+                            const double* pr_diag, lsx_gmres_stats* stats) {
+    if (!A || !b || !x) return LSX_EINVAL;
+    lsx_ctx* ctx = A->ctx;
+    const int64_t N = A->N;
+    if (blocksize < 1 || N % blocksize != 0)
+        return fail(ctx, LSX_EINVAL, "lsx_gmres_solve_batched: blocksize %lld must divide N = %lld", (long long)blocksize,
+                    (long long)N);
+    if (memory < 1 || maxiters < 0)
+        return fail(ctx, LSX_EINVAL, "lsx_gmres_solve_batched: memory >= 1, maxiters >= 0");
+    DeviceGuard g(ctx);
+    const int64_t nb = N / blocksize;
+    GmresParams p{memory, restart ? 1 : 0, maxiters, 0.0, 0.0};
+    GmresPlan plan;
+    int rc = gmres_plan(ctx, blocksize, p, &plan);
+    if (rc) return rc;
+    if (plan.P != 1)
+        return fail(ctx, LSX_EINVAL, "lsx_gmres_solve_batched: blocks of more than %d rows are not supported", 16 * GBS);
+    const hipStream_t st = A->stream;
+    // local columns (once per block size), after checking that A is block diagonal
+    if (A->local_bs != blocksize) {
+        if (!A->col_local) LSX_HIP(ctx, hipMalloc(&A->col_local, sizeof(int32_t) * std::max<int64_t>(A->nnz, 1)));
+        LSX_HIP(ctx, hipMemsetAsync(A->bflag, 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_local_cols, dim3((unsigned)std::min<int64_t>(cdiv(N, 256), 4096)), dim3(256), 0, st, N,
+                           blocksize, A->rowptr, A->col, A->col_local, A->bflag);
+        int bad = 0;
+        LSX_HIP(ctx, hipMemcpyAsync(&bad, A->bflag, sizeof(int), hipMemcpyDeviceToHost, st));
+        LSX_HIP(ctx, hipStreamSynchronize(st));
+        if (bad) return fail(ctx, LSX_EINVAL, "lsx_gmres_solve_batched: A has entries outside its %lld x %lld diagonal blocks",
+                             (long long)blocksize, (long long)blocksize);
+        A->local_bs = blocksize;
+    }
+    // buffers for nb blocks
+    const int64_t rw = gmres_red_words(1);
+    if (nb > A->bcap) {
+        (void)hipFree(A->bred);
+        (void)hipFree(A->bitems);
+        (void)hipFree(A->bfirst);
+        (void)hipFree(A->bout);
+        A->bred = nullptr;
+        A->bitems = nullptr;
+        A->bfirst = nullptr;
+        A->bout = nullptr;
+        A->bcap = 0;
+        LSX_HIP(ctx, hipMalloc(&A->bred, sizeof(uint64_t) * rw * nb));
+        LSX_HIP(ctx, hipMalloc(&A->bitems, sizeof(GmresArgs) * nb));
+        LSX_HIP(ctx, hipMalloc(&A->bfirst, sizeof(int) * (nb + 1)));
+        LSX_HIP(ctx, hipMalloc(&A->bout, sizeof(GmresOut) * nb));
+        A->bcap = nb;
+    }
+    if (!A->bticket) LSX_HIP(ctx, hipMalloc(&A->bticket, 256));
+    const size_t need = sizeof(double) * (size_t)N * (size_t)(plan.cap + 2);
+    if (need > A->ws_bytes) {
+        (void)hipFree(A->ws);
+        A->ws = nullptr;
+        A->ws_bytes = 0;
+        if (hipMalloc(&A->ws, need) != hipSuccess)
+            return fail(ctx, LSX_ENOMEM, "gmres: cannot allocate %zu bytes of Krylov workspace", need);
+        A->ws_bytes = need;
+    }
+    const size_t vb = sizeof(double) * N;
+    LSX_HIP(ctx, hipMemcpyAsync(A->db, b, vb, hipMemcpyDefault, st));
+    if (pl_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpl, pl_diag, vb, hipMemcpyDefault, st));
+    if (pr_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpr, pr_diag, vb, hipMemcpyDefault, st));
+    // the stopping tolerance of the whole batch: eps = abstol + reltol ||Pl \ b||_2
+    // (src/simplegmres.jl:110-129: beta is the norm of the whole vector), from the host copy
+    double beta = 0.0;
+    {
+        std::vector<double> hb(N), hp;
+        LSX_HIP(ctx, hipMemcpyAsync(hb.data(), A->db, vb, hipMemcpyDeviceToHost, st));
+        if (pl_diag) {
+            hp.resize(N);
+            LSX_HIP(ctx, hipMemcpyAsync(hp.data(), A->dpl, vb, hipMemcpyDeviceToHost, st));
+        }
+        LSX_HIP(ctx, hipStreamSynchronize(st));
+        for (int64_t i = 0; i < N; ++i) {
+            const double r = pl_diag ? hb[i] / hp[i] : hb[i];
+            beta += r * r;
+        }
+        beta = std::sqrt(beta);
+    }
+    const double eps = abstol + reltol * beta;
+    const int64_t memv = std::max<int64_t>(1, std::min<int64_t>(memory, maxiters));
+    std::vector<GmresArgs> items(nb);
+    std::vector<int> first(nb + 1);
+    for (int64_t i = 0; i < nb; ++i) {
+        GmresArgs& a = items[i];
+        const int64_t r0 = i * blocksize;
+        a.N = blocksize;
+        a.rowptr = A->rowptr + r0;
+        a.col = A->col_local;
+        a.val = A->val;
+        a.b = A->db + r0;
+        a.x = A->dx + r0;
+        a.pl = pl_diag ? A->dpl + r0 : nullptr;
+        a.pr = pr_diag ? A->dpr + r0 : nullptr;
+        a.V = A->ws + r0 * (plan.cap + 2);
+        a.xr = a.V + blocksize * (plan.cap + 1);
+        a.memory = (int)memv;
+        a.restart = restart ? 1 : 0;
+        a.cap = plan.cap;
+        a.maxiters = maxiters;
+        a.abstol = eps;
+        a.reltol = 0.0;
+        a.P = 1;
+        a.red = A->bred + i * rw;
+        a.bar = reinterpret_cast<uint32_t*>(a.red + 4);
+        a.out = A->bout + i;
+        a.timeout = ctx->wall_khz * 20000;
+        first[i] = (int)i;
+    }
+    first[nb] = (int)nb;
+    LSX_HIP(ctx, hipMemcpyAsync(A->bitems, items.data(), sizeof(GmresArgs) * nb, hipMemcpyHostToDevice, st));
+    LSX_HIP(ctx, hipMemcpyAsync(A->bfirst, first.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    const int pool = (int)std::min<int64_t>(nb, 4 * (int64_t)ctx->num_cus);
+    rc = gmres_launch_multi(ctx, st, A->bitems, A->bfirst, (int)nb, pool, plan.rpt, plan.dynlds, A->bticket);
+    if (rc) return rc;
+    std::vector<GmresOut> outs(nb);
+    LSX_HIP(ctx, hipMemcpyAsync(outs.data(), A->bout, sizeof(GmresOut) * nb, hipMemcpyDeviceToHost, st));
+    LSX_HIP(ctx, hipMemcpyAsync(x, A->dx, vb, hipMemcpyDefault, st));
+    LSX_HIP(ctx, hipStreamSynchronize(st));
+    // the batch's result: iterations and rNorm of its slowest block; Infeasible if any block is,
+    // else Success if every block is solved, else MaxIters (src/simplegmres.jl:603-606)
+    int64_t iters = 0;
+    double resid = 0.0;
+    bool any_inf = false, all_ok = true;
+    for (const GmresOut& o : outs) {
+        if (o.err) return fail(ctx, LSX_ETIMEOUT, "lsx_gmres_solve_batched: a device-side wait exceeded its bound");
+        iters = std::max<int64_t>(iters, o.iters);
+        resid = std::max(resid, o.resid);
+        any_inf = any_inf || o.retcode == LSX_RET_INFEASIBLE;
+        all_ok = all_ok && o.retcode == LSX_RET_SUCCESS;
+    }
+    if (stats) {
+        stats->retcode = any_inf ? LSX_RET_INFEASIBLE : (all_ok ? LSX_RET_SUCCESS : LSX_RET_MAXITERS);
+        stats->pad = 0;
+        stats->iters = iters;
+        stats->resid = resid;
     }
     return LSX_OK;
 }

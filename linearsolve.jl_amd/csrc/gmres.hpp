@@ -75,6 +75,17 @@ struct lsx_csr {
     double* dx = nullptr;
     double* dpl = nullptr;
     double* dpr = nullptr;
+    // batched (block-diagonal) solves: column indices local to each row's block, per-block
+    // exchange words, launch tables
+    int32_t* col_local = nullptr;
+    int64_t local_bs = 0;        // block size col_local was made for (0: none)
+    uint64_t* bred = nullptr;    // nblocks x gmres_red_words(1)
+    lsx::GmresArgs* bitems = nullptr;
+    int* bfirst = nullptr;
+    lsx::GmresOut* bout = nullptr;
+    uint32_t* bticket = nullptr;
+    int* bflag = nullptr;        // off-block entry found
+    int64_t bcap = 0;            // blocks the batched buffers hold
 };
 
 namespace lsx {
@@ -91,4 +102,7 @@ int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out);
 int gmres_launch_multi(lsx_ctx* ctx, hipStream_t st, const GmresArgs* items, const int* first, int nitems,
                        int pool, int rpt, size_t dynlds, uint32_t* ticket);
 int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st);
+// Checks a device CSR structure on stream st: *dflag (device) becomes nonzero when rowptr or the
+// column indices are out of range.  The caller reads the flag after the stream.
+int csr_validate(hipStream_t st, int64_t N, int64_t nnz, const int32_t* rowptr, const int32_t* col, int* dflag);
 }  // namespace lsx

@@ -277,6 +277,8 @@ struct GmresLane {
     GmresArgs* h_items = nullptr;  // pinned
     int* h_first = nullptr;        // pinned
     GmresOut* h_out = nullptr;     // pinned
+    int* h_flags = nullptr;        // pinned: CSR checks of a launch's requests
+    int* d_flags = nullptr;
     GmresArgs* d_items = nullptr;
     int* d_first = nullptr;
     GmresOut* d_out = nullptr;
@@ -509,35 +511,57 @@ void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
     const hipStream_t st = L.stream;
     const int cnt = (int)group.size();
     int rc = LSX_OK;
-    int total = 0;
-    size_t lds = 0;
     std::vector<GmresBlock> blk(cnt);
+    std::vector<char> has_pl(cnt, 0), bad(cnt, 0);
+    bool user = false;
+    // inputs into each request's heap block; a caller's CSR is checked before it is used
     for (int i = 0; i < cnt && rc == LSX_OK; ++i) {
         Job* j = group[i];
         const lsx_request& r = j->req;
         const GmresBlock g = gmres_block(s->heap.base + j->off, *j);
         blk[i] = g;
         const int64_t N = j->N;
-        bool has_pl = r.pl_diag != nullptr;
+        has_pl[i] = r.pl_diag != nullptr;
         if (r.synth) {
             hipLaunchKernelGGL(k_gen_convdiff, dim3((unsigned)std::min<int64_t>(cdiv(N, 256), 4096)), dim3(256), 0, st,
                                (int)r.grid_m, r.pe, r.seed, g.rowptr, g.col, g.val, g.b, g.pl);
-            has_pl = true;
+            has_pl[i] = 1;
         } else if (hipMemcpyAsync(g.rowptr, r.rowptr, sizeof(int32_t) * (N + 1), hipMemcpyDefault, st) != hipSuccess ||
                    (j->nnz > 0 && (hipMemcpyAsync(g.col, r.colind, sizeof(int32_t) * j->nnz, hipMemcpyDefault, st) != hipSuccess ||
                                    hipMemcpyAsync(g.val, r.vals, sizeof(double) * j->nnz, hipMemcpyDefault, st) != hipSuccess)) ||
                    hipMemcpyAsync(g.b, r.b, sizeof(double) * N, hipMemcpyDefault, st) != hipSuccess ||
-                   (has_pl && hipMemcpyAsync(g.pl, r.pl_diag, sizeof(double) * N, hipMemcpyDefault, st) != hipSuccess)) {
+                   (has_pl[i] && hipMemcpyAsync(g.pl, r.pl_diag, sizeof(double) * N, hipMemcpyDefault, st) != hipSuccess) ||
+                   csr_validate(st, N, j->nnz, g.rowptr, g.col, L.d_flags + i) != LSX_OK) {
             rc = LSX_EHIP;
+        } else {
+            user = true;
         }
-        GmresArgs& a = L.h_items[i];
-        a.N = N;
+    }
+    if (rc == LSX_OK && user) {
+        if (hipMemcpyAsync(L.h_flags, L.d_flags, sizeof(int) * cnt, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = LSX_EHIP;
+        for (int i = 0; i < cnt && rc == LSX_OK; ++i) bad[i] = !group[i]->req.synth && L.h_flags[i] != 0;
+    }
+    // the launch over the requests that passed
+    std::vector<int> idx;
+    int total = 0, widest = 0;
+    size_t lds = 0;
+    for (int i = 0; i < cnt && rc == LSX_OK; ++i) {
+        if (bad[i]) continue;
+        Job* j = group[i];
+        const lsx_request& r = j->req;
+        const GmresBlock& g = blk[i];
+        const int m = (int)idx.size();
+        idx.push_back(i);
+        GmresArgs& a = L.h_items[m];
+        a.N = j->N;
         a.rowptr = g.rowptr;
         a.col = g.col;
         a.val = g.val;
         a.b = g.b;
         a.x = g.x;
-        a.pl = has_pl ? g.pl : nullptr;
+        a.pl = has_pl[i] ? g.pl : nullptr;
         a.pr = nullptr;
         a.V = g.V;
         a.xr = g.xr;
@@ -550,37 +574,39 @@ void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
         a.P = j->plan.P;
         a.red = g.red;
         a.bar = reinterpret_cast<uint32_t*>(g.red + 4 * (int64_t)j->plan.P);
-        a.out = L.d_out + i;
+        a.out = L.d_out + m;
         a.timeout = ctx->wall_khz * 20000;  // 20 s
-        L.h_first[i] = total;
+        L.h_first[m] = total;
         total += j->plan.P;
+        widest = std::max(widest, j->plan.P);
         lds = std::max(lds, j->plan.dynlds);
     }
-    L.h_first[cnt] = total;
-    if (rc == LSX_OK &&
-        (hipMemcpyAsync(L.d_items, L.h_items, sizeof(GmresArgs) * cnt, hipMemcpyHostToDevice, st) != hipSuccess ||
-         hipMemcpyAsync(L.d_first, L.h_first, sizeof(int) * (cnt + 1), hipMemcpyHostToDevice, st) != hipSuccess))
-        rc = LSX_EHIP;
-    int widest = 0;
-    for (Job* j : group) widest = std::max(widest, j->plan.P);
-    const int pool = std::min(total, std::max(widest, s->gmres_pool));
-    if (rc == LSX_OK) rc = gmres_launch_multi(ctx, st, L.d_items, L.d_first, cnt, pool, group[0]->plan.rpt, lds, L.d_ticket);
-    if (rc == LSX_OK &&
-        hipMemcpyAsync(L.h_out, L.d_out, sizeof(GmresOut) * cnt, hipMemcpyDeviceToHost, st) != hipSuccess)
-        rc = LSX_EHIP;
-    for (int i = 0; i < cnt && rc == LSX_OK; ++i)
-        if (hipMemcpyAsync(group[i]->x_out, blk[i].x, sizeof(double) * group[i]->N, hipMemcpyDefault, st) != hipSuccess)
+    const int nl = (int)idx.size();
+    if (rc == LSX_OK && nl > 0) {
+        L.h_first[nl] = total;
+        const int pool = std::min(total, std::max(widest, s->gmres_pool));
+        if (hipMemcpyAsync(L.d_items, L.h_items, sizeof(GmresArgs) * nl, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(L.d_first, L.h_first, sizeof(int) * (nl + 1), hipMemcpyHostToDevice, st) != hipSuccess)
             rc = LSX_EHIP;
-    if (hipStreamSynchronize(st) != hipSuccess && rc == LSX_OK) rc = LSX_EHIP;
-    for (int i = 0; i < cnt; ++i) {
-        Job* j = group[i];
-        j->res.status = rc;
-        if (rc == LSX_OK) {
-            if (L.h_out[i].err) j->res.status = LSX_ETIMEOUT;
-            j->res.retcode = L.h_out[i].retcode;
-            j->res.iters = L.h_out[i].iters;
-            j->res.resid = L.h_out[i].resid;
+        if (rc == LSX_OK)
+            rc = gmres_launch_multi(ctx, st, L.d_items, L.d_first, nl, pool, group[idx[0]]->plan.rpt, lds, L.d_ticket);
+        if (rc == LSX_OK &&
+            hipMemcpyAsync(L.h_out, L.d_out, sizeof(GmresOut) * nl, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = LSX_EHIP;
+        for (int m = 0; m < nl && rc == LSX_OK; ++m) {
+            Job* j = group[idx[m]];
+            if (hipMemcpyAsync(j->x_out, blk[idx[m]].x, sizeof(double) * j->N, hipMemcpyDefault, st) != hipSuccess)
+                rc = LSX_EHIP;
         }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && rc == LSX_OK) rc = LSX_EHIP;
+    for (int i = 0; i < cnt; ++i) group[i]->res.status = rc != LSX_OK ? rc : (bad[i] ? LSX_EINVAL : LSX_OK);
+    for (int m = 0; m < nl && rc == LSX_OK; ++m) {
+        Job* j = group[idx[m]];
+        if (L.h_out[m].err) j->res.status = LSX_ETIMEOUT;
+        j->res.retcode = L.h_out[m].retcode;
+        j->res.iters = L.h_out[m].iters;
+        j->res.resid = L.h_out[m].resid;
     }
 }
 
@@ -737,6 +763,8 @@ int lanes_alloc(lsx_serve* s, int nd, int ng) {
             hipHostMalloc(&L.h_items, sizeof(GmresArgs) * MAXG, hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc(&L.h_first, sizeof(int) * (MAXG + 1), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc(&L.h_out, sizeof(GmresOut) * MAXG, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&L.h_flags, sizeof(int) * MAXG, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&L.d_flags, sizeof(int) * MAXG) != hipSuccess ||
             hipMalloc(&L.d_items, sizeof(GmresArgs) * MAXG) != hipSuccess ||
             hipMalloc(&L.d_first, sizeof(int) * (MAXG + 1)) != hipSuccess ||
             hipMalloc(&L.d_out, sizeof(GmresOut) * MAXG) != hipSuccess || hipMalloc(&L.d_ticket, 256) != hipSuccess)
@@ -758,6 +786,8 @@ void lanes_free(lsx_serve* s) {
         if (L.h_items) (void)hipHostFree(L.h_items);
         if (L.h_first) (void)hipHostFree(L.h_first);
         if (L.h_out) (void)hipHostFree(L.h_out);
+        if (L.h_flags) (void)hipHostFree(L.h_flags);
+        (void)hipFree(L.d_flags);
         (void)hipFree(L.d_items);
         (void)hipFree(L.d_first);
         (void)hipFree(L.d_out);

@@ -136,3 +136,124 @@ def test_matrix_rhs_rejected(lsx):
     A = lsx.CSRMatrix.from_scipy(sp.identity(4, format="csr"))
     with pytest.raises(ValueError):
         lsx.init(lsx.LinearProblem(A, np.ones((4, 2))), lsx.MI355XSimpleGMRES())
+
+
+# ---- batched layout (SimpleGMRES(blocksize > 0), src/simplegmres.jl:190-373)
+
+def test_reference_6x6_blocksize_2(lsx):
+    # test/Core/basictests.jl:651-710 with blocksize = 2: the matrices keep only their 2 x 2
+    # diagonal blocks; resolve with b mutated / replaced / unchanged at rtol 1e-13, and the
+    # preconditioned cases at rtol 1e-8 with Success
+    nr = 6
+    bs = 2
+    mask = np.array([[(i - 1) // bs == (j - 1) // bs for j in range(1, nr + 1)] for i in range(1, nr + 1)])
+    Ar = np.array([[10.0 + i if i == j else 0.3 * (i + j) for j in range(1, nr + 1)] for i in range(1, nr + 1)]) * mask
+    br = np.arange(1.0, nr + 1)
+    br2 = np.array([0.3 * i + 1 for i in range(1, nr + 1)])
+    for restart in (True, False):
+        for mode in ("mutated", "replaced", "unchanged"):
+            A = lsx.CSRMatrix.from_scipy(sp.csr_matrix(Ar))
+            cache = lsx.init(lsx.LinearProblem(A, br.copy()), lsx.MI355XSimpleGMRES(restart=restart, blocksize=bs),
+                             abstol=1e-12, reltol=1e-12)
+            np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(Ar, br), rtol=1e-13)
+            if mode == "mutated":
+                cache.b[:] = br2
+                expected = br2
+            elif mode == "replaced":
+                cache.b = br2.copy()
+                expected = br2
+            else:
+                expected = br
+            np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(Ar, expected), rtol=1e-13)
+    Ap = np.array([[10.0 + 5 * i if i == j else 0.3 * (i + j) for j in range(1, nr + 1)] for i in range(1, nr + 1)]) * mask
+    d = np.diag(Ap)
+    for restart in (True, False):
+        for Pl, Pr in ((d, None), (None, d), (np.sqrt(d), np.sqrt(d))):
+            A = lsx.CSRMatrix.from_scipy(sp.csr_matrix(Ap))
+            sol = lsx.solve(lsx.LinearProblem(A, br.copy()), lsx.MI355XSimpleGMRES(restart=restart, blocksize=bs),
+                            Pl=Pl, Pr=Pr, abstol=1e-12, reltol=1e-12, maxiters=100)
+            assert sol.retcode == lsx.ReturnCode.Success
+            np.testing.assert_allclose(sol.u, np.linalg.solve(Ap, br), rtol=1e-8)
+
+
+@pytest.mark.parametrize("bs,nblocks", [(16, 40), (100, 12), (512, 4), (1024, 3)])
+def test_batched_blocks_match_oracle_per_block(lsx, bs, nblocks):
+    # every block is its own SimpleGMRES: each block's result equals the oracle's SimpleGMRES on
+    # that block alone with the batch's tolerance eps = abstol + reltol ||Pl \ b||_2 (same
+    # retcode, iterations within 1, solution within 1e-6 relative); the batch reports its slowest
+    # block's iterations
+    print(f"batched gmres bs={bs} blocks={nblocks}", flush=True)
+    P = problems(lsx)
+    rng = np.random.default_rng(bs)
+    m = int(round(bs ** 0.5))
+    blocks = []
+    for k in range(nblocks):
+        if m * m == bs:
+            rp, ci, v = P.convection_diffusion(m, 1.0 + k % 5)
+            blocks.append(sp.csr_matrix((v, ci, rp), shape=(bs, bs)))
+        else:
+            B = sp.random(bs, bs, density=4.0 / bs, random_state=int(rng.integers(1 << 30))) + 4.0 * sp.eye(bs)
+            blocks.append(sp.csr_matrix(B))
+    S = sp.block_diag(blocks, format="csr")
+    S.sort_indices()
+    N = bs * nblocks
+    b = rng.standard_normal(N)
+    d = S.diagonal()
+    reltol, maxit = 1e-8, 2000
+    A = lsx.CSRMatrix.from_scipy(S)
+    sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XSimpleGMRES(memory=20, blocksize=bs), Pl=d, reltol=reltol,
+                    maxiters=maxit)
+    eps = np.sqrt(np.finfo(float).eps) + reltol * np.linalg.norm(b / d)
+    worst_iters = 0
+    for k in range(nblocks):
+        sl = slice(k * bs, (k + 1) * bs)
+        Bk = blocks[k]
+        Bk.sort_indices()
+        ref = O.gmres(Bk.indptr, Bk.indices, Bk.data, b[sl], memory=20, restart=True, abstol=eps, reltol=0.0,
+                      maxiters=maxit, pl=d[sl])
+        assert ref["retcode"] == O.RET_SUCCESS
+        worst_iters = max(worst_iters, ref["iters"])
+        np.testing.assert_allclose(sol.u[sl], ref["u"], rtol=0, atol=1e-6 * np.abs(ref["u"]).max())
+        assert np.linalg.norm((b[sl] - Bk @ sol.u[sl]) / d[sl]) <= 1.5 * eps
+    assert sol.retcode == lsx.ReturnCode.Success
+    assert abs(sol.iters - worst_iters) <= 1
+
+
+def test_batched_rejects_bad_layouts(lsx):
+    # blocksize must divide N (simplegmres.jl:422); entries outside the diagonal blocks are an error
+    S = sp.csr_matrix(np.diag(np.arange(1.0, 7.0)) + np.diag(np.ones(5), 1))
+    A = lsx.CSRMatrix.from_scipy(S)
+    with pytest.raises(ValueError):
+        lsx.solve(lsx.LinearProblem(A, np.ones(6)), lsx.MI355XSimpleGMRES(blocksize=4))
+    with pytest.raises(lsx.LSXError):
+        lsx.solve(lsx.LinearProblem(A, np.ones(6)), lsx.MI355XSimpleGMRES(blocksize=2))
+
+
+def test_malformed_csr_is_rejected(lsx):
+    # the structure check (rowptr bounds and monotonicity, column range) runs before any solver
+    # kernel reads the matrix: bad input is an error, not an out-of-range access
+    good_rp = np.array([0, 2, 3, 5], dtype=np.int32)
+    good_ci = np.array([0, 1, 1, 0, 2], dtype=np.int32)
+    vals = np.ones(5)
+    cases = [
+        (np.array([0, 2, 3, 9], dtype=np.int32), good_ci),      # rowptr[N] != nnz
+        (np.array([0, 3, 2, 5], dtype=np.int32), good_ci),      # decreasing row
+        (np.array([1, 2, 3, 5], dtype=np.int32), good_ci),      # rowptr[0] != 0
+        (good_rp, np.array([0, 1, 1, 0, 7], dtype=np.int32)),   # column out of range
+        (good_rp, np.array([0, -1, 1, 0, 2], dtype=np.int32)),  # negative column
+    ]
+    for rp, ci in cases:
+        with pytest.raises(lsx.LSXError):
+            lsx._CSRHandle(lsx.Context.get(0), lsx.CSRMatrix(rp, ci, vals, (3, 3)))
+    lsx._CSRHandle(lsx.Context.get(0), lsx.CSRMatrix(good_rp, good_ci, vals, (3, 3)))
+    # the serving lane checks a request's CSR the same way: the bad request fails alone
+    srv = lsx.Server(hbm_budget=1 << 30)
+    bad = srv.submit_gmres(lsx.CSRMatrix(cases[3][0], cases[3][1], vals, (3, 3)), np.ones(3), reltol=1e-8, maxiters=50)
+    good = srv.submit_gmres(lsx.CSRMatrix(good_rp, good_ci, vals + np.array([3.0, 0, 3.0, 0, 3.0]), (3, 3)),
+                            np.ones(3), reltol=1e-10, maxiters=50)
+    srv.drain()
+    rb, rg = srv.poll(bad), srv.poll(good)
+    from linearsolve_jl_amd import _abi
+    assert rb["status"] == _abi.LSX_EINVAL
+    assert rg["status"] == 0 and rg["retcode"] == lsx.ReturnCode.Success
+    srv.close()
