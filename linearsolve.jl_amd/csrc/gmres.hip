@@ -280,22 +280,36 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
                         q(i) = a.pl ? wv / a.pl[r] : wv;
                     }
                 }
-                // modified Gram-Schmidt, the axpy of basis vector i fused with the dot of i + 1
+                // modified Gram-Schmidt, the axpy of basis vector i fused with the dot of i + 1;
+                // the rows of the next basis vector are loaded while the all-reduce travels
                 double h = 0.0;
+                double pre[RPT];
+#pragma unroll
+                for (int i = 0; i < RPT; ++i) {
+                    const int64_t r = g.row(i);
+                    pre[i] = (r < N) ? a.V[r] : 0.0;
+                }
                 for (int ii = 1; ii <= k + 1; ++ii) {
                     part = 0.0;
-                    const double* vi = a.V + (int64_t)(ii - 1) * N;
-#pragma unroll 2
+#pragma unroll
                     for (int i = 0; i < RPT; ++i) {
                         const int64_t r = g.row(i);
                         if (r < N) {
                             if (ii > 1) q(i) = q(i) - h * vk(i);
                             if (ii <= k) {
-                                vk(i) = vi[r];
+                                vk(i) = pre[i];
                                 part += vk(i) * q(i);
                             } else {
                                 part += q(i) * q(i);
                             }
+                        }
+                    }
+                    if (ii < k) {
+                        const double* vn = a.V + (int64_t)ii * N;
+#pragma unroll
+                        for (int i = 0; i < RPT; ++i) {
+                            const int64_t r = g.row(i);
+                            pre[i] = (r < N) ? vn[r] : 0.0;
                         }
                     }
                     const double red = g.allreduce(part);
