@@ -623,6 +623,7 @@ static void csr_free(lsx_csr* A) {
     (void)hipFree(A->bout);
     (void)hipFree(A->bticket);
     (void)hipFree(A->bflag);
+    (void)hipFree(A->bpart);
     (void)hipFree(A->rowptr);
     (void)hipFree(A->col);
     (void)hipFree(A->val);
@@ -636,6 +637,46 @@ static void csr_free(lsx_csr* A) {
     (void)hipFree(A->dpr);
     if (A->stream) (void)hipStreamDestroy(A->stream);
 }
+
+namespace lsx {
+
+// Stopping tolerance of a batched solve, on the device: partial sums of (b / pl)^2 over a fixed
+// grid of NPART blocks, then every block of k_batch_tol sums the partials in the same fixed order
+// and writes eps = abstol + reltol sqrt(sum) into its share of the launch table (one value for all
+// blocks, the same rounding on every call).
+constexpr int NPART = 256;
+
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double* b, const double* pl, int64_t N, double* part) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (int64_t)NPART * 256) {
+        const double r = pl ? b[i] / pl[i] : b[i];
+        acc = fma(r, r, acc);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_batch_tol(const double* part, double abstol, double reltol, GmresArgs* items,
+                                                   int64_t nitems) {
+    __shared__ double s[NPART];
+    s[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int w = NPART / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double eps = abstol + reltol * sqrt(s[0]);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nitems; i += (int64_t)gridDim.x * 256)
+        items[i].abstol = eps;
+}
+
+}  // namespace lsx
 
 extern "C" {
 
@@ -782,6 +823,7 @@ int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, doub
         LSX_HIP(ctx, hipMalloc(&A->bfirst, sizeof(int) * (nb + 1)));
         LSX_HIP(ctx, hipMalloc(&A->bout, sizeof(GmresOut) * nb));
         A->bcap = nb;
+        A->bkey[0] = 0;  // new table buffers: rebuild
     }
     if (!A->bticket) LSX_HIP(ctx, hipMalloc(&A->bticket, 256));
     const size_t need = sizeof(double) * (size_t)N * (size_t)(plan.cap + 2);
@@ -793,60 +835,55 @@ int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, doub
             return fail(ctx, LSX_ENOMEM, "gmres: cannot allocate %zu bytes of Krylov workspace", need);
         A->ws_bytes = need;
     }
+    if (!A->bpart) LSX_HIP(ctx, hipMalloc(&A->bpart, sizeof(double) * NPART));
     const size_t vb = sizeof(double) * N;
     LSX_HIP(ctx, hipMemcpyAsync(A->db, b, vb, hipMemcpyDefault, st));
     if (pl_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpl, pl_diag, vb, hipMemcpyDefault, st));
     if (pr_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpr, pr_diag, vb, hipMemcpyDefault, st));
-    // the stopping tolerance of the whole batch: eps = abstol + reltol ||Pl \ b||_2
-    // (src/simplegmres.jl:110-129: beta is the norm of the whole vector), from the host copy
-    double beta = 0.0;
-    {
-        std::vector<double> hb(N), hp;
-        LSX_HIP(ctx, hipMemcpyAsync(hb.data(), A->db, vb, hipMemcpyDeviceToHost, st));
-        if (pl_diag) {
-            hp.resize(N);
-            LSX_HIP(ctx, hipMemcpyAsync(hp.data(), A->dpl, vb, hipMemcpyDeviceToHost, st));
-        }
-        LSX_HIP(ctx, hipStreamSynchronize(st));
-        for (int64_t i = 0; i < N; ++i) {
-            const double r = pl_diag ? hb[i] / hp[i] : hb[i];
-            beta += r * r;
-        }
-        beta = std::sqrt(beta);
-    }
-    const double eps = abstol + reltol * beta;
+    // the launch table, rebuilt when what it holds changes; its tolerance is written on the device
     const int64_t memv = std::max<int64_t>(1, std::min<int64_t>(memory, maxiters));
-    std::vector<GmresArgs> items(nb);
-    std::vector<int> first(nb + 1);
-    for (int64_t i = 0; i < nb; ++i) {
-        GmresArgs& a = items[i];
-        const int64_t r0 = i * blocksize;
-        a.N = blocksize;
-        a.rowptr = A->rowptr + r0;
-        a.col = A->col_local;
-        a.val = A->val;
-        a.b = A->db + r0;
-        a.x = A->dx + r0;
-        a.pl = pl_diag ? A->dpl + r0 : nullptr;
-        a.pr = pr_diag ? A->dpr + r0 : nullptr;
-        a.V = A->ws + r0 * (plan.cap + 2);
-        a.xr = a.V + blocksize * (plan.cap + 1);
-        a.memory = (int)memv;
-        a.restart = restart ? 1 : 0;
-        a.cap = plan.cap;
-        a.maxiters = maxiters;
-        a.abstol = eps;
-        a.reltol = 0.0;
-        a.P = 1;
-        a.red = A->bred + i * rw;
-        a.bar = reinterpret_cast<uint32_t*>(a.red + 4);
-        a.out = A->bout + i;
-        a.timeout = ctx->wall_khz * 20000;
-        first[i] = (int)i;
+    const int64_t key[8] = {blocksize, (pl_diag ? 1 : 0) | (pr_diag ? 2 : 0), memv, restart ? 1 : 0, maxiters,
+                            (int64_t)(uintptr_t)A->ws, (int64_t)(uintptr_t)A->bred, plan.cap};
+    if (!std::equal(key, key + 8, A->bkey)) {
+        std::vector<GmresArgs> items(nb);
+        std::vector<int> first(nb + 1);
+        for (int64_t i = 0; i < nb; ++i) {
+            GmresArgs& a = items[i];
+            const int64_t r0 = i * blocksize;
+            a.N = blocksize;
+            a.rowptr = A->rowptr + r0;
+            a.col = A->col_local;
+            a.val = A->val;
+            a.b = A->db + r0;
+            a.x = A->dx + r0;
+            a.pl = pl_diag ? A->dpl + r0 : nullptr;
+            a.pr = pr_diag ? A->dpr + r0 : nullptr;
+            a.V = A->ws + r0 * (plan.cap + 2);
+            a.xr = a.V + blocksize * (plan.cap + 1);
+            a.memory = (int)memv;
+            a.restart = restart ? 1 : 0;
+            a.cap = plan.cap;
+            a.maxiters = maxiters;
+            a.abstol = 0.0;
+            a.reltol = 0.0;
+            a.P = 1;
+            a.red = A->bred + i * rw;
+            a.bar = reinterpret_cast<uint32_t*>(a.red + 4);
+            a.out = A->bout + i;
+            a.timeout = ctx->wall_khz * 20000;
+            first[i] = (int)i;
+        }
+        first[nb] = (int)nb;
+        LSX_HIP(ctx, hipMemcpyAsync(A->bitems, items.data(), sizeof(GmresArgs) * nb, hipMemcpyHostToDevice, st));
+        LSX_HIP(ctx, hipMemcpyAsync(A->bfirst, first.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+        LSX_HIP(ctx, hipStreamSynchronize(st));  // the host tables are freed on return
+        std::copy(key, key + 8, A->bkey);
     }
-    first[nb] = (int)nb;
-    LSX_HIP(ctx, hipMemcpyAsync(A->bitems, items.data(), sizeof(GmresArgs) * nb, hipMemcpyHostToDevice, st));
-    LSX_HIP(ctx, hipMemcpyAsync(A->bfirst, first.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
+    // the stopping tolerance of the whole batch: eps = abstol + reltol ||Pl \ b||_2
+    // (src/simplegmres.jl:110-129: beta is the norm of the whole vector)
+    hipLaunchKernelGGL(k_sumsq_partial, dim3(NPART), dim3(256), 0, st, A->db, pl_diag ? A->dpl : nullptr, N, A->bpart);
+    hipLaunchKernelGGL(k_batch_tol, dim3((unsigned)std::min<int64_t>(cdiv(nb, 256), 64)), dim3(256), 0, st, A->bpart,
+                       abstol, reltol, A->bitems, nb);
     const int pool = (int)std::min<int64_t>(nb, 4 * (int64_t)ctx->num_cus);
     rc = gmres_launch_multi(ctx, st, A->bitems, A->bfirst, (int)nb, pool, plan.rpt, plan.dynlds, A->bticket);
     if (rc) return rc;

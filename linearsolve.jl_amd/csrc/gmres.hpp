@@ -86,6 +86,8 @@ struct lsx_csr {
     uint32_t* bticket = nullptr;
     int* bflag = nullptr;        // off-block entry found
     int64_t bcap = 0;            // blocks the batched buffers hold
+    double* bpart = nullptr;     // partial sums of the stopping norm
+    int64_t bkey[8] = {};        // parameters the device launch table was built for
 };
 
 namespace lsx {
