@@ -11,11 +11,13 @@
 //
 // Scheduling: systems are sorted by decreasing size on the host (longest first); a persistent
 // grid of workgroups takes them from an atomic counter, one workgroup per system.
-// Per system (BT threads): right-looking blocked LU with panels of BW columns held in LDS; rows
-// are never moved in memory: a per-system permutation perm[] (logical row -> physical row)
-// absorbs the interchanges, so a swap outside the panel is a swap of two perm entries.  The
-// trailing update A22 -= L21 U12 runs on MFMA (FP64 16x16x4 or FP32 16x16x4) with L21 from LDS
-// and U12 from L2.  The panel width BW (8 or 16) is chosen per system from a size-binned tile
+// Per system (BT threads): right-looking blocked LU with panels of BW columns held in LDS.  A
+// panel's row interchanges are applied in memory to the columns right of it as U12 is formed
+// (one thread per column moves the at most 2 BW rows involved), so the trailing update
+// A22 -= L21 U12 reads and writes contiguous rows; it runs on MFMA (FP64 16x16x4 or FP32 16x16x4,
+// 64 x 32 blocks per wave) with L21 from LDS and U12 from L2.  The L columns take the
+// interchanges of later panels in one gather after the last panel (like dlaswp on the left, but
+// once).  The panel width BW (8 or 16) is chosen per system from a size-binned tile
 // table (set by the autotuner through lsx_batch_set_tiles).  Triangular solves go by blocks of 64
 // rows: the diagonal block is staged in LDS and solved by one wave with shuffles, the rows
 // beyond it are updated by all threads.
@@ -40,6 +42,7 @@ constexpr int BT = 256;            // threads per system
 constexpr int BNMAX = 512;         // largest system
 constexpr int PBUF = BNMAX * 17;   // panel buffer in doubles: 512 x (16 + 1) or 512 x (8 + 1)
 constexpr int SB = 64;             // row block of the triangular solves (one wave)
+constexpr int RBT = 4, CBT = 2;    // trailing-update block of a wave, in 16 x 16 tiles
 
 // Panel-width index of a system without a tile table.
 __host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : 1; }
@@ -71,8 +74,13 @@ struct BatchShared {
     int wrow[BT / 64];
     int prow;
     double red[2][BT / 64];
+    int chg[16];               // rows below the panel that its interchanges move
+    int nchg;
     double xs[BNMAX];          // solve vector
-    double xr[BNMAX];          // FP32 path: the refined solution
+    union {
+        double xr[BNMAX];      // FP32 path: the refined solution (after the factorization)
+        int piv[BNMAX];        // 0-based swap sequence (during the factorization)
+    };
     int perm[BNMAX];
 };
 
@@ -111,7 +119,11 @@ __device__ __forceinline__ void block_max2(BatchShared& sh, double& a, double& b
     __syncthreads();
 }
 
-// LU with partial pivoting of one n x n system (column-major, in place, rows through sh.perm).
+// LU with partial pivoting of one n x n system (column-major, in place).  Each panel's row
+// interchanges reach the columns right of it while U12 is formed, so the trailing update reads
+// and writes rows in place (coalesced); the columns left of it (L) take all later interchanges in
+// one gather at the end, which also leaves the system's permutation (logical row -> original row)
+// in sh.perm.  On return A holds L and U in LAPACK's row order.
 template <typename T, int BW>
 __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -119,10 +131,11 @@ __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ip
     for (int j0 = 0; j0 < n; j0 += BW) {
         const int jb = min(BW, n - j0);
         const int m = n - j0;
-        // load the panel (logical rows j0..n-1 through perm)
+        // rows j0..n-1 of columns j0..n-1 are in place: the panel's permutation starts as identity
+        for (int i = j0 + tid; i < n; i += BT) sh.perm[i] = i;
         for (int idx = tid; idx < m * BW; idx += BT) {
             const int r = idx % m, c = idx / m;
-            P[r][c] = (c < jb) ? A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] : T(0);
+            P[r][c] = (c < jb) ? A[(int64_t)(j0 + c) * n + j0 + r] : T(0);
         }
         __syncthreads();
         for (int c = 0; c < jb; ++c) {
@@ -149,6 +162,7 @@ __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ip
                     if (bbetter(sh.wkey[i], sh.wrow[i], bk, br)) { bk = sh.wkey[i]; br = sh.wrow[i]; }
                 sh.prow = br;
                 const int k = j0 + c;
+                sh.piv[k] = j0 + br;
                 if (ipiv) ipiv[vo + k] = (int64_t)(j0 + br) + 1;
                 if (br != c) {
                     const int t = sh.perm[k];
@@ -177,58 +191,136 @@ __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ip
             }
             __syncthreads();
         }
-        // write the panel back (physical rows through the updated perm)
+        // the panel, rows in LAPACK's order
         for (int idx = tid; idx < m * jb; idx += BT) {
             const int r = idx % m, c = idx / m;
-            A[(int64_t)(j0 + c) * n + sh.perm[j0 + r]] = P[r][c];
+            A[(int64_t)(j0 + c) * n + j0 + r] = P[r][c];
         }
         const int c1 = j0 + jb;
         if (c1 < n) {
-            // U12 = L11^{-1} A12, one thread per column
+            // the rows below the panel that its swaps move: those it swaps with (at most jb)
+            if (tid == 0) {
+                int cnt = 0;
+                for (int k = j0; k < c1; ++k) {
+                    const int p = sh.piv[k];
+                    bool skip = p < c1;
+                    for (int t = 0; t < cnt && !skip; ++t) skip = sh.chg[t] == p;
+                    if (!skip) sh.chg[cnt++] = p;
+                }
+                sh.nchg = cnt;
+            }
+            __syncthreads();
+            const int nchg = sh.nchg;
+            // here jb == BW (only the last panel is narrower, and it has no columns right of it).
+            // Right of the panel, one thread per column: the interchanges (rows j0..c1-1 and the
+            // moved rows below, every source read before any row is written; slots past nchg
+            // repeat the last move), then U12 = L11^{-1} A12.  Offsets within a system fit 32 bits.
             for (int col = c1 + tid; col < n; col += BT) {
-                T u[BW];
+                const int co = col * n;
+                T u[BW], w[BW];
 #pragma unroll
-                for (int i = 0; i < BW; ++i) u[i] = (i < jb) ? A[(int64_t)col * n + sh.perm[j0 + i]] : T(0);
+                for (int i = 0; i < BW; ++i) u[i] = A[co + sh.perm[j0 + i]];
+                if (nchg > 0) {
+#pragma unroll
+                    for (int t = 0; t < BW; ++t) w[t] = A[co + sh.perm[sh.chg[min(t, nchg - 1)]]];
+#pragma unroll
+                    for (int t = 0; t < BW; ++t) A[co + sh.chg[min(t, nchg - 1)]] = w[t];
+                }
 #pragma unroll
                 for (int kk = 0; kk < BW; ++kk) {
 #pragma unroll
                     for (int i = kk + 1; i < BW; ++i) u[i] = fma(-P[i][kk], u[kk], u[i]);
-                    __builtin_amdgcn_sched_barrier(0);  // L11 reads column by column, not all hoisted
+                    __builtin_amdgcn_sched_barrier(0);  // L11 read column by column, not all at once
                 }
 #pragma unroll
-                for (int i = 0; i < BW; ++i)
-                    if (i < jb) A[(int64_t)col * n + sh.perm[j0 + i]] = u[i];
+                for (int i = 0; i < BW; ++i) A[co + j0 + i] = u[i];
             }
             __syncthreads();
-            // A22 -= L21 U12 on MFMA: 16 x 16 tiles, waves round-robin
-            const int mr = n - c1, nc = n - c1;
-            const int tr = (mr + 15) / 16, tc = (nc + 15) / 16;
-            for (int t = wave; t < tr * tc; t += BT / 64) {
-                const int rb = (t % tr) * 16, cb = (t / tr) * 16;
-                const int lr = rb + (lane & 15);  // local row of C
-                const bool rok = lr < mr;
-                const int prow = rok ? sh.perm[c1 + lr] : 0;
-                typename BMma<T>::acc_t acc;
+            // A22 -= L21 U12 on MFMA: blocks of RBT x CBT 16 x 16 tiles, waves round-robin
+            const int mr = n - c1;
+            const int tbr = (mr + 16 * RBT - 1) / (16 * RBT), tbc = (mr + 16 * CBT - 1) / (16 * CBT);
+            T* A22 = A + c1 * n + c1;
+            const T* U12 = A + c1 * n + j0;
+            for (int t = wave; t < tbr * tbc; t += BT / 64) {
+                const int rb = (t % tbr) * (16 * RBT), cb = (t / tbr) * (16 * CBT);
+                int lr[RBT];
+                bool rok[RBT];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int lc = cb + BMma<T>::ccol(lane, q);
-                    acc[q] = (rok && lc < nc) ? A[(int64_t)(c1 + lc) * n + prow] : T(0);
+                for (int a2 = 0; a2 < RBT; ++a2) {
+                    lr[a2] = rb + 16 * a2 + (lane & 15);
+                    rok[a2] = lr[a2] < mr;
                 }
+                typename BMma<T>::acc_t acc[RBT][CBT];
+#pragma unroll
+                for (int a2 = 0; a2 < RBT; ++a2)
+#pragma unroll
+                    for (int b2 = 0; b2 < CBT; ++b2)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
+                            acc[a2][b2][q] = (rok[a2] && lc < mr) ? A22[lc * n + lr[a2]] : T(0);
+                        }
 #pragma unroll
                 for (int k4 = 0; k4 < BW / 4; ++k4) {
                     const int kk = k4 * 4 + (lane >> 4);
-                    const int lc = cb + (lane & 15);
-                    const T xv = (lc < nc && kk < jb) ? -A[(int64_t)(c1 + lc) * n + sh.perm[j0 + kk]] : T(0);
-                    const T yv = (rok && kk < jb) ? P[jb + lr][kk] : T(0);
-                    acc = BMma<T>::mma(xv, yv, acc);
+                    T xv[CBT], yv[RBT];
+#pragma unroll
+                    for (int b2 = 0; b2 < CBT; ++b2) {
+                        const int lc = cb + 16 * b2 + (lane & 15);
+                        xv[b2] = (lc < mr) ? -U12[lc * n + kk] : T(0);
+                    }
+#pragma unroll
+                    for (int a2 = 0; a2 < RBT; ++a2) yv[a2] = rok[a2] ? P[BW + lr[a2]][kk] : T(0);
+#pragma unroll
+                    for (int a2 = 0; a2 < RBT; ++a2)
+#pragma unroll
+                        for (int b2 = 0; b2 < CBT; ++b2) acc[a2][b2] = BMma<T>::mma(xv[b2], yv[a2], acc[a2][b2]);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int lc = cb + BMma<T>::ccol(lane, q);
-                    if (rok && lc < nc) A[(int64_t)(c1 + lc) * n + prow] = acc[q];
-                }
+                for (int a2 = 0; a2 < RBT; ++a2)
+#pragma unroll
+                    for (int b2 = 0; b2 < CBT; ++b2)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
+                            if (rok[a2] && lc < mr) A22[lc * n + lr[a2]] = acc[a2][b2][q];
+                        }
             }
         }
+        __syncthreads();
+    }
+    // The columns of panel p hold their rows below the panel in the order after panel p; later
+    // panels swapped those rows.  S (kept in sh.perm) is the composition of the swaps after panel
+    // p, built from the last panel back (row i of the final order is row S(i) of that order).
+    for (int i = tid; i < n; i += BT) sh.perm[i] = i;
+    __syncthreads();
+    T* Q = reinterpret_cast<T*>(sh.pbuf);
+    bool moved = false;  // some later swap exchanges two different rows
+    for (int p = (n - 1) / BW; p >= 0; --p) {
+        const int j0 = p * BW, jb = min(BW, n - j0), c1 = j0 + jb, m2 = n - c1;
+        if (moved) {
+            for (int idx = tid; idx < m2 * jb; idx += BT) {
+                const int r = idx % m2, c = idx / m2;
+                Q[idx] = A[(int64_t)(j0 + c) * n + sh.perm[c1 + r]];
+            }
+            __syncthreads();
+            for (int idx = tid; idx < m2 * jb; idx += BT) {
+                const int r = idx % m2, c = idx / m2;
+                A[(int64_t)(j0 + c) * n + c1 + r] = Q[idx];
+            }
+        }
+        // S := tau(j0) o ... o tau(c1 - 1) o S, tau(k) exchanging rows k and piv[k]
+        int pk[BW];
+#pragma unroll
+        for (int k = 0; k < BW; ++k) pk[k] = sh.piv[min(j0 + k, c1 - 1)];
+        for (int i = j0 + tid; i < n; i += BT) {
+            int x = sh.perm[i];
+#pragma unroll
+            for (int k = BW - 1; k >= 0; --k)
+                if (k < jb) x = (x == j0 + k) ? pk[k] : (x == pk[k] ? j0 + k : x);
+            sh.perm[i] = x;
+        }
+        for (int k = j0; k < c1; ++k) moved = moved || sh.piv[k] != k;
         __syncthreads();
     }
 }
@@ -243,7 +335,7 @@ __device__ void solve_sys(BatchShared& sh, const T* A, int n) {
         const int kb = min(SB, n - k0);
         for (int idx = tid; idx < kb * kb; idx += BT) {
             const int r = idx % kb, c = idx / kb;
-            D[r][c] = (double)A[(int64_t)(k0 + c) * n + sh.perm[k0 + r]];
+            D[r][c] = (double)A[(int64_t)(k0 + c) * n + k0 + r];
         }
         __syncthreads();
         if (wave == 0) {
@@ -257,8 +349,7 @@ __device__ void solve_sys(BatchShared& sh, const T* A, int n) {
         __syncthreads();
         for (int i = k0 + kb + tid; i < n; i += BT) {
             double s = sh.xs[i];
-            const int pi = sh.perm[i];
-            for (int k = k0; k < k0 + kb; ++k) s = fma(-(double)A[(int64_t)k * n + pi], sh.xs[k], s);
+            for (int k = k0; k < k0 + kb; ++k) s = fma(-(double)A[(int64_t)k * n + i], sh.xs[k], s);
             sh.xs[i] = s;
         }
         __syncthreads();
@@ -268,7 +359,7 @@ __device__ void solve_sys(BatchShared& sh, const T* A, int n) {
         const int k0 = max(0, k1 - SB), kb = k1 - k0;
         for (int idx = tid; idx < kb * kb; idx += BT) {
             const int r = idx % kb, c = idx / kb;
-            D[r][c] = (double)A[(int64_t)(k0 + c) * n + sh.perm[k0 + r]];
+            D[r][c] = (double)A[(int64_t)(k0 + c) * n + k0 + r];
         }
         __syncthreads();
         if (wave == 0) {
@@ -283,8 +374,7 @@ __device__ void solve_sys(BatchShared& sh, const T* A, int n) {
         __syncthreads();
         for (int i = tid; i < k0; i += BT) {
             double s = sh.xs[i];
-            const int pi = sh.perm[i];
-            for (int k = k0; k < k1; ++k) s = fma(-(double)A[(int64_t)k * n + pi], sh.xs[k], s);
+            for (int k = k0; k < k1; ++k) s = fma(-(double)A[(int64_t)k * n + i], sh.xs[k], s);
             sh.xs[i] = s;
         }
         __syncthreads();
@@ -307,8 +397,17 @@ __global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
         const int64_t vo = a.voff[sys];
         int64_t info = 0;
         const double* A64 = a.A64 ? a.A64 + a.aoff[sys] : nullptr;
-        if (A64) {  // FP32 path: working copy of the FP64 matrix
-            for (int64_t i = tid; i < (int64_t)n * n; i += BT) A[i] = (T)A64[i];
+        double an = 0.0;  // FP32 path: this thread's rows' largest sum |a_ij| (||A||_inf)
+        if (A64) {  // FP32 path: working copy of the FP64 matrix, by rows, with the row sums
+            for (int i = tid; i < n; i += BT) {
+                double rs = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    const double v = A64[j * n + i];
+                    A[j * n + i] = (T)v;
+                    rs += fabs(v);
+                }
+                an = fmax(an, rs);
+            }
         }
         for (int i = tid; i < n; i += BT) sh.perm[i] = i;
         __syncthreads();
@@ -321,13 +420,8 @@ __global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
         solve_sys<T>(sh, A, n);
         if (A64) {
             // FP64 refinement
-            double an = 0.0, bn = 0.0;
-            for (int i = tid; i < n; i += BT) {
-                double s = 0.0;
-                for (int j = 0; j < n; ++j) s += fabs(A64[(int64_t)j * n + i]);
-                an = fmax(an, s);
-                bn = fmax(bn, fabs(a.b[vo + i]));
-            }
+            double bn = 0.0;
+            for (int i = tid; i < n; i += BT) bn = fmax(bn, fabs(a.b[vo + i]));
             block_max2(sh, an, bn);
             for (int i = tid; i < n; i += BT) sh.xr[i] = sh.xs[i];
             __syncthreads();
