@@ -72,7 +72,7 @@ struct BatchShared {
     double pbuf[PBUF];         // panel (factor type, row stride BW + 1) / staged diagonal blocks
     double wkey[BT / 64];
     int wrow[BT / 64];
-    int prow;
+    double rowbuf[2][16];      // pivot row and the row it displaces (panel columns)
     double red[2][BT / 64];
     int chg[16];               // rows below the panel that its interchanges move
     int nchg;
@@ -83,6 +83,12 @@ struct BatchShared {
     };
     int perm[BNMAX];
 };
+
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ __forceinline__ bool bbetter(double ka, int ra, double kb, int rb) {
     return ka > kb || (ka == kb && ra < rb);
@@ -119,33 +125,26 @@ __device__ __forceinline__ void block_max2(BatchShared& sh, double& a, double& b
     __syncthreads();
 }
 
-// LU with partial pivoting of one n x n system (column-major, in place).  Each panel's row
-// interchanges reach the columns right of it while U12 is formed, so the trailing update reads
-// and writes rows in place (coalesced); the columns left of it (L) take all later interchanges in
-// one gather at the end, which also leaves the system's permutation (logical row -> original row)
-// in sh.perm.  On return A holds L and U in LAPACK's row order.
-template <typename T, int BW>
-__device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    T (*P)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(sh.pbuf);
-    for (int j0 = 0; j0 < n; j0 += BW) {
-        const int jb = min(BW, n - j0);
-        const int m = n - j0;
-        // rows j0..n-1 of columns j0..n-1 are in place: the panel's permutation starts as identity
-        for (int i = j0 + tid; i < n; i += BT) sh.perm[i] = i;
-        for (int idx = tid; idx < m * BW; idx += BT) {
-            const int r = idx % m, c = idx / m;
-            P[r][c] = (c < jb) ? A[(int64_t)(j0 + c) * n + j0 + r] : T(0);
-        }
-        __syncthreads();
-        for (int c = 0; c < jb; ++c) {
-            // pivot: first row of max |a| among logical rows j0+c..n-1 (NaN never wins)
+// Column C of a panel held in registers (v: thread tid's logical rows tid and tid + BT): pivot
+// search, the interchange of rows C and the pivot row through LDS, elimination below row C.
+// Two barriers per column.  Instantiated for C = 0..BW-1 so that v is indexed by constants.
+template <typename T, int BW, int C>
+__device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m, int j0, int jb, int64_t vo,
+                                          int64_t* ipiv, int64_t& info) {
+    if constexpr (C < BW) {
+        const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+        if (C < jb) {  // uniform: only the last panel is narrower
+            // pivot: first row of max |a| among logical rows C..m-1 (NaN never wins)
             double key = -1.0;
             int krow = 0x7fffffff;
-            for (int r = c + tid; r < m; r += BT) {
-                const double t = fabs((double)P[r][c]);
-                const double kk = (t == t) ? t : 0.0;
-                if (bbetter(kk, r, key, krow)) { key = kk; krow = r; }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int r = tid + k * BT;
+                if (r >= C && r < m) {
+                    const double t = fabs((double)v[k][C]);
+                    const double kk = (t == t) ? t : 0.0;
+                    if (bbetter(kk, r, key, krow)) { key = kk; krow = r; }
+                }
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -155,47 +154,105 @@ __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ip
             }
             if (lane == 0) { sh.wkey[wave] = key; sh.wrow[wave] = krow; }
             __syncthreads();
+            double bk = sh.wkey[0];
+            int pr = sh.wrow[0];
+#pragma unroll
+            for (int i = 1; i < BT / 64; ++i)
+                if (bbetter(sh.wkey[i], sh.wrow[i], bk, pr)) { bk = sh.wkey[i]; pr = sh.wrow[i]; }
+            // the pivot row and (when they differ) row C through LDS; row C is thread C's first row
+            if (tid == (pr & (BT - 1))) {
+#pragma unroll
+                for (int q = 0; q < BW; ++q) sh.rowbuf[0][q] = (double)(pr < BT ? v[0][q] : v[1][q]);
+            }
+            if (pr != C && tid == C) {
+#pragma unroll
+                for (int q = 0; q < BW; ++q) sh.rowbuf[1][q] = (double)v[0][q];
+            }
             if (tid == 0) {
-                double bk = sh.wkey[0];
-                int br = sh.wrow[0];
-                for (int i = 1; i < BT / 64; ++i)
-                    if (bbetter(sh.wkey[i], sh.wrow[i], bk, br)) { bk = sh.wkey[i]; br = sh.wrow[i]; }
-                sh.prow = br;
-                const int k = j0 + c;
-                sh.piv[k] = j0 + br;
-                if (ipiv) ipiv[vo + k] = (int64_t)(j0 + br) + 1;
-                if (br != c) {
+                const int k = j0 + C;
+                sh.piv[k] = j0 + pr;
+                if (ipiv) ipiv[vo + k] = (int64_t)(j0 + pr) + 1;
+                if (pr != C) {
                     const int t = sh.perm[k];
-                    sh.perm[k] = sh.perm[j0 + br];
-                    sh.perm[j0 + br] = t;
+                    sh.perm[k] = sh.perm[j0 + pr];
+                    sh.perm[j0 + pr] = t;
                 }
             }
             __syncthreads();
-            const int pr = sh.prow;
-            if (pr != c && tid < BW) {
-                const T t = P[c][tid];
-                P[c][tid] = P[pr][tid];
-                P[pr][tid] = t;
-            }
-            __syncthreads();
-            const T pv = P[c][c];
-            if (pv == T(0) && info == 0) info = j0 + c + 1;
-            for (int r = c + 1 + tid; r < m; r += BT) {
-                T l = P[r][c];
-                if (pv != T(0)) {
-                    if (fabs((double)pv) >= (double)std::numeric_limits<T>::min()) l *= (T(1) / pv);
-                    else l /= pv;
+            T prow[BW];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) prow[q] = (T)sh.rowbuf[0][q];
+            if (pr != C) {
+                if (tid == C) {
+#pragma unroll
+                    for (int q = 0; q < BW; ++q) v[0][q] = prow[q];
                 }
-                P[r][c] = l;
-                for (int q = c + 1; q < jb; ++q) P[r][q] = fma(-l, P[c][q], P[r][q]);
+                if (tid == (pr & (BT - 1))) {
+#pragma unroll
+                    for (int q = 0; q < BW; ++q) {
+                        const T t = (T)sh.rowbuf[1][q];
+                        if (pr < BT) v[0][q] = t;
+                        else v[1][q] = t;
+                    }
+                }
             }
-            __syncthreads();
+            const T pv = prow[C];
+            if (pv == T(0) && info == 0) info = j0 + C + 1;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int r = tid + k * BT;
+                if (r > C && r < m) {
+                    T l = v[k][C];
+                    if (pv != T(0)) {
+                        if (fabs((double)pv) >= (double)std::numeric_limits<T>::min()) l *= (T(1) / pv);
+                        else l /= pv;
+                    }
+                    v[k][C] = l;
+#pragma unroll
+                    for (int q = C + 1; q < BW; ++q) v[k][q] = fma(-l, prow[q], v[k][q]);
+                }
+            }
+    }
+        panel_col<T, BW, C + 1>(sh, v, m, j0, jb, vo, ipiv, info);
+    }
+}
+
+// LU with partial pivoting of one n x n system (column-major, in place).  Each panel's row
+// interchanges reach the columns right of it while U12 is formed, so the trailing update reads
+// and writes rows in place (coalesced); the columns left of it (L) take all later interchanges in
+// one gather at the end, which also leaves the system's permutation (logical row -> original row)
+// in sh.perm.  On return A holds L and U in LAPACK's row order.
+template <typename T, int BW>
+__device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T (*P)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(sh.pbuf);
+    for (int j0 = 0; j0 < n; j0 += BW) {
+        const int jb = min(BW, n - j0);
+        const int m = n - j0;
+        // rows j0..n-1 of columns j0..n-1 are in place: the panel's permutation starts as identity
+        for (int i = j0 + tid; i < n; i += BT) sh.perm[i] = i;
+        // the panel in registers: thread tid holds logical rows tid and tid + BT (m <= 2 BT)
+        T v[2][BW];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int r = tid + k * BT;
+#pragma unroll
+            for (int q = 0; q < BW; ++q) v[k][q] = (r < m && q < jb) ? A[(j0 + q) * n + j0 + r] : T(0);
         }
-        // the panel, rows in LAPACK's order
-        for (int idx = tid; idx < m * jb; idx += BT) {
-            const int r = idx % m, c = idx / m;
-            A[(int64_t)(j0 + c) * n + j0 + r] = P[r][c];
+        panel_col<T, BW, 0>(sh, v, m, j0, jb, vo, ipiv, info);
+        // the factored panel, rows in LAPACK's order: to LDS (L11, L21 for the updates) and to A
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int r = tid + k * BT;
+            if (r < m) {
+#pragma unroll
+                for (int q = 0; q < BW; ++q) {
+                    P[r][q] = v[k][q];
+                    if (q < jb) A[(j0 + q) * n + j0 + r] = v[k][q];
+                }
+            }
         }
+        __syncthreads();
         const int c1 = j0 + jb;
         if (c1 < n) {
             // the rows below the panel that its swaps move: those it swaps with (at most jb)
@@ -327,7 +384,7 @@ __device__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ip
 
 // sh.xs := U^{-1} L^{-1} sh.xs (sh.xs already holds P b), arithmetic in FP64.
 template <typename T>
-__device__ void solve_sys(BatchShared& sh, const T* A, int n) {
+__device__ __forceinline__ void solve_sys(BatchShared& sh, const T* A, int n) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double (*D)[SB + 1] = reinterpret_cast<double (*)[SB + 1]>(sh.pbuf);
     // forward: unit lower L
@@ -389,14 +446,16 @@ __global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
     while (true) {
         if (tid == 0) s_item = (int)atomicAdd(a.counter, 1u);
         __syncthreads();
-        const int64_t item = s_item;
+        // the system's scalars are wave-uniform (held in scalar registers)
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= a.count) break;
-        const int sys = a.order[item];
-        const int n = a.n[sys];
-        T* A = reinterpret_cast<T*>(a.A) + a.aoff[sys];
-        const int64_t vo = a.voff[sys];
+        const int sys = __builtin_amdgcn_readfirstlane(a.order[item]);
+        const int n = __builtin_amdgcn_readfirstlane(a.n[sys]);
+        const int64_t ao = uniform64(a.aoff[sys]);
+        T* A = reinterpret_cast<T*>(a.A) + ao;
+        const int64_t vo = uniform64(a.voff[sys]);
         int64_t info = 0;
-        const double* A64 = a.A64 ? a.A64 + a.aoff[sys] : nullptr;
+        const double* A64 = a.A64 ? a.A64 + ao : nullptr;
         double an = 0.0;  // FP32 path: this thread's rows' largest sum |a_ij| (||A||_inf)
         if (A64) {  // FP32 path: working copy of the FP64 matrix, by rows, with the row sums
             for (int i = tid; i < n; i += BT) {
