@@ -180,6 +180,15 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
                     double abstol, double reltol, int64_t maxiters, const double* pl_diag,
                     const double* pr_diag, lsx_gmres_stats* stats);
 
+/* The largest Krylov dimension a solve on A can hold: memory with restart, or maxiters without
+ * (the packed triangle R and the Givens pairs live in the solving workgroups' LDS beside their
+ * rows of q and the basis, so it depends on N: 128 for N <= 512, about 70 for N > 2048).
+ * Replaces the growth of the workspace that SimpleGMRES and KrylovJL_GMRES do without restart
+ * (src/simplegmres.jl:222-235 _gmres_grow_pass!/_gmres_grow_krylov!, Krylov.jl's gmres with
+ * restart = false under src/iterative_wrappers.jl:693): a caller asks first and restarts at
+ * this dimension when maxiters exceeds it. */
+int lsx_gmres_max_memory(lsx_csr* A, int32_t* memory);
+
 /* SimpleGMRES with blocksize > 0 (the batched layout, src/simplegmres.jl:190-373): A is block
  * diagonal with N / blocksize uniform blocks, and every block runs its own GMRES.  Each block
  * stops when its residual estimate reaches eps = abstol + reltol ||Pl \ b||_2 (the norm of the

@@ -26,7 +26,7 @@ from ._abi import LSX_BF16, LSX_F32, LSX_F64, LSXError, Context
 __all__ = [
     "ReturnCode", "LinearProblem", "LinearCache", "LinearSolution", "init", "solve", "solve_",
     "MI355XLUFactorization", "MI355XMixedLUFactorization", "MI355XBatchedLUFactorization",
-    "MI355XSimpleGMRES", "BlockDiagonal", "CSRMatrix", "LSXError", "Context", "Server",
+    "MI355XSimpleGMRES", "MI355XKrylovJL_GMRES", "BlockDiagonal", "CSRMatrix", "LSXError", "Context", "Server",
     "default_tol", "default_algorithm", "OperatorAssumptions", "resize_",
 ]
 
@@ -172,9 +172,9 @@ def default_algorithm(A, b, assumptions=None):
     build's algorithms, with the conditioning thresholds of the reference's dense route
     (src/default.jl:411-477: LU for Well- and IllConditioned, QR for VeryIllConditioned, SVD for
     SuperIllConditioned) instead of its GPU route's QR for the default IllConditioned
-    (:277-306), and a GMRES Krylov method for sparse operators (:327-343):
+    (:277-306), and KrylovJL_GMRES for square operators without a direct solver (:320-343):
       dense square (numpy or torch)  -> MI355XLUFactorization
-      CSRMatrix                      -> MI355XSimpleGMRES (the device GMRES)
+      CSRMatrix                      -> MI355XKrylovJL_GMRES (the device GMRES)
       BlockDiagonal                  -> MI355XBatchedLUFactorization (blockwise LU)
     QR, SVD and the non-square Krylov methods are not built here: those choices raise."""
     assumptions = assumptions or OperatorAssumptions()
@@ -183,7 +183,7 @@ def default_algorithm(A, b, assumptions=None):
     if isinstance(A, CSRMatrix):
         if A.shape[0] != A.shape[1]:
             raise NotImplementedError("non-square sparse systems (LSMR / CRAIGMR) are not built")
-        return MI355XSimpleGMRES()
+        return MI355XKrylovJL_GMRES()
     shape = tuple(A.shape)
     issq = (shape[0] == shape[1]) if assumptions.issq is None else assumptions.issq
     if not issq or len(shape) != 2 or shape[0] != shape[1]:
@@ -206,6 +206,9 @@ def init(prob, alg=None, *, alias_A=False, alias_b=False, abstol=None, reltol=No
     if isinstance(alg, MI355XSimpleGMRES) and len(b.shape) == 2:
         # src/common.jl:614-625: Krylov methods take a vector right-hand side only
         raise ValueError("SimpleGMRES does not support a matrix right-hand side b")
+    if isinstance(alg, MI355XKrylovJL_GMRES) and len(b.shape) == 2:
+        # the reference hands a matrix b to Krylov.jl's block GMRES (src/iterative_wrappers.jl:705-706)
+        raise NotImplementedError("block GMRES for a matrix right-hand side is not built")
     u = _empty_like_rhs(b)
     if _is_torch(u):
         u.zero_()
@@ -237,8 +240,8 @@ def solve_(cache, adjoint=False):
         cache.cacheval.solve_adjoint(cache.b, cache.u)
     else:
         At = _adjoint_of(cache.A)
-        if isinstance(alg, MI355XSimpleGMRES):
-            # SimpleGMRES has a right slot: Pl' = adjoint(Pr), Pr' = adjoint(Pl) (diagonal: themselves)
+        if isinstance(alg, (MI355XSimpleGMRES, MI355XKrylovJL_GMRES)):
+            # the GMRES methods have a right slot: Pl' = adjoint(Pr), Pr' = adjoint(Pl) (diagonal: themselves)
             sol = solve(LinearProblem(At, _copy(cache.b)), alg, abstol=cache.abstol, reltol=cache.reltol,
                         maxiters=cache.maxiters, Pl=cache.Pr, Pr=cache.Pl, device=cache.ctx.device)
         else:
@@ -681,11 +684,99 @@ class MI355XSimpleGMRES:
                 cache.abstol, cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
                 pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve_batched")
         else:
+            memory, restart = _device_krylov_dims(cache, self.memory, bool(self.restart))
             cache.ctx.check(_abi.lib().lsx_gmres_solve(
-                cache.cacheval.h, pb, _ptr_out(cache.u), self.memory, int(bool(self.restart)), cache.abstol,
+                cache.cacheval.h, pb, _ptr_out(cache.u), memory, int(restart), cache.abstol,
                 cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
                 pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve")
         return LinearSolution(cache.u, st.resid, ReturnCode(st.retcode), int(st.iters), cache)
+
+
+def _device_krylov_dims(cache, memory, restart):
+    """(memory, restart) for the device solve.  Without restart the reference grows the Krylov
+    basis up to maxiters vectors (src/simplegmres.jl:222-235; Krylov.jl's gmres likewise); the
+    device holds at most lsx_gmres_max_memory of them, so a solve that could need more restarts
+    every that many iterations instead."""
+    if restart:
+        return memory, True
+    lim = ctypes.c_int32()
+    cache.ctx.check(_abi.lib().lsx_gmres_max_memory(cache.cacheval.h, ctypes.byref(lim)), "lsx_gmres_max_memory")
+    if max(memory, cache.maxiters) <= lim.value:
+        return memory, False
+    return int(lim.value), True
+
+
+class MI355XKrylovJL_GMRES:
+    """Drop-in for KrylovJL_GMRES (src/iterative_wrappers.jl:200-228, the KrylovJL struct
+    :142-162, its solve! :625-768) on a CSR matrix, on the device GMRES.
+
+    KrylovJL_GMRES calls Krylov.gmres! from the Krylov.jl package, which the reference does not
+    vendor.  Its published algorithm is the one SimpleGMRES restates (src/simplegmres.jl:186-192:
+    "the Arnoldi loop, the Givens bookkeeping, the stopping criteria and the restart logic"):
+    MGS Arnoldi on M A N, Givens QR of the Hessenberg matrix, stop when the residual estimate
+    reaches atol + rtol ||M r0|| (or breakdown / itmax), which the device kernel computes.
+    Options, as the reference maps them (:211-221, :680-693):
+      gmres_restart = 0  -> memory = min(20, n), no restart (the basis grows to maxiters)
+      gmres_restart > 0  -> memory = gmres_restart, restart every gmres_restart iterations
+      memory             -> overrides the size derived from gmres_restart
+      precs(A, p)        -> (Pl, Pr), diagonal preconditioners (vectors, or None for identity),
+                            applied with ldiv (Pl \\ x) or, with ldiv=False, by multiplication
+    The solution: retcode Success when solved, MaxIters when the iterations ran out, Failure
+    otherwise (an inconsistent system); iters = niter; resid = the last residual norm (:730-766).
+    warm_start=Auto starts from zero, as in standalone LinearSolve (:669-673)."""
+
+    needs_concrete_A = True
+    needs_square_A = True
+
+    def __init__(self, gmres_restart=0, memory=None, precs=None, ldiv=True, warm_start="Auto", window=0):
+        if gmres_restart < 0:
+            raise ValueError("gmres_restart must be >= 0")
+        if warm_start not in ("Auto", None):
+            raise ValueError("only WarmStart.Auto (a cold start) is supported")
+        self.gmres_restart = int(gmres_restart)
+        self.memory = memory
+        self.precs = precs
+        self.ldiv = bool(ldiv)
+        self.window = window  # accepted and unused for GMRES (:222-223)
+
+    def init_cacheval(self, cache):
+        return None
+
+    def _preconditioners(self, cache, n):
+        if self.precs is None:
+            return _diag_of(cache.Pl, n), _diag_of(cache.Pr, n)
+        Pl, Pr = self.precs(cache.A, None)
+        pl, pr = _diag_of(Pl, n), _diag_of(Pr, n)
+        if not self.ldiv:  # M * x with M = Diagonal(m) is Diagonal(1 ./ m) \ x
+            pl = None if pl is None else np.ascontiguousarray(1.0 / pl)
+            pr = None if pr is None else np.ascontiguousarray(1.0 / pr)
+        return pl, pr
+
+    def solve_(self, cache):
+        A = cache.A
+        if not isinstance(A, CSRMatrix):
+            A = CSRMatrix.from_scipy(A)
+            object.__setattr__(cache, "_A", A)
+        n = A.shape[0]
+        if cache.isfresh or cache.cacheval is None:
+            cache.cacheval = _CSRHandle(cache.ctx, A)
+            object.__setattr__(cache, "isfresh", False)
+        pl, pr = self._preconditioners(cache, n)
+        kb, pb = _colmajor(cache.b)
+        if self.memory is not None:
+            memory = int(self.memory)
+        else:
+            memory = self.gmres_restart if self.gmres_restart > 0 else min(20, n)
+        memory, restart = _device_krylov_dims(cache, memory, self.gmres_restart > 0)
+        st = _abi.GmresStats()
+        cache.ctx.check(_abi.lib().lsx_gmres_solve(
+            cache.cacheval.h, pb, _ptr_out(cache.u), memory, int(restart), cache.abstol, cache.reltol,
+            cache.maxiters, pl.ctypes.data if pl is not None else None, pr.ctypes.data if pr is not None else None,
+            ctypes.byref(st)), "lsx_gmres_solve")
+        rc = ReturnCode(st.retcode)
+        if rc == ReturnCode.Infeasible:
+            rc = ReturnCode.Failure
+        return LinearSolution(cache.u, st.resid, rc, int(st.iters), cache)
 
 
 # -------------------------------------------------------------------------------------------------

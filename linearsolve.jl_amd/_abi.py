@@ -97,6 +97,7 @@ SIGNATURES = [
     ("lsx_csr_destroy", ctypes.c_int, [_P]),
     ("lsx_csr_spmv", ctypes.c_int, [_P, _P, _P]),
     ("lsx_gmres_solve", ctypes.c_int, [_P, _P, _P, _I32, _I32, _D, _D, _I64, _P, _P, ctypes.POINTER(GmresStats)]),
+    ("lsx_gmres_max_memory", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     ("lsx_gmres_solve_batched", ctypes.c_int, [_P, _I64, _P, _P, _I32, _I32, _D, _D, _I64, _P, _P,
                                                ctypes.POINTER(GmresStats)]),
     ("lsx_serve_create", ctypes.c_int, [_P, ctypes.c_size_t, _I32, ctypes.POINTER(_P)]),

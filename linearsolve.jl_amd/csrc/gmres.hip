@@ -484,6 +484,21 @@ int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? LSX_OK : LSX_EHIP;
 }
 
+// Dynamic LDS of a solve: the packed triangle R, the Givens pairs and z, q and the basis slice.
+static size_t gmres_dynlds(int64_t cap, int rpt) {
+    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
+}
+constexpr size_t GMRES_LDS_MAX = 150 * 1024;
+
+// The largest Krylov dimension whose solve state fits a workgroup, for a matrix of N rows.
+int gmres_cap_limit(int64_t N) {
+    int P = 1;
+    const int rpt = pick_rpt(N, P);
+    int cap = GMRES_MAX_CAP;
+    while (cap > 1 && gmres_dynlds(cap, rpt) > GMRES_LDS_MAX) --cap;
+    return cap;
+}
+
 int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out) {
     int P = 1;
     const int rpt = pick_rpt(N, P);
@@ -493,8 +508,8 @@ int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out) {
     if (cap > GMRES_MAX_CAP)
         return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld exceeds %d (restart = %d)", (long long)cap,
                     GMRES_MAX_CAP, p.restart);
-    const size_t dynlds = sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
-    if (dynlds > 150 * 1024)
+    const size_t dynlds = gmres_dynlds(cap, rpt);
+    if (dynlds > GMRES_LDS_MAX)
         return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld too large for N = %lld", (long long)cap, (long long)N);
     out->P = P;
     out->rpt = rpt;
@@ -770,6 +785,12 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
         stats->iters = A->hout->iters;
         stats->resid = A->hout->resid;
     }
+    return LSX_OK;
+}
+
+int lsx_gmres_max_memory(lsx_csr* A, int32_t* memory) {
+    if (!A || !memory) return LSX_EINVAL;
+    *memory = gmres_cap_limit(A->N);
     return LSX_OK;
 }
 

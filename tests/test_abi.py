@@ -89,8 +89,8 @@ def test_convection_diffusion_generator():
 
 
 def test_default_algorithm_routes():
-    # solve(prob) without an algorithm: dense square -> LU, CSR -> SimpleGMRES, BlockDiagonal ->
-    # batched LU; the QR / SVD choices of the reference (non-square, VeryIll / SuperIllConditioned)
+    # solve(prob) without an algorithm: dense square -> LU, CSR -> KrylovJL_GMRES (the reference's
+    # choice for square operators, src/default.jl:320-343), BlockDiagonal -> batched LU; the QR / SVD choices of the reference (non-square, VeryIll / SuperIllConditioned)
     # are not built and raise.  Host logic only (no device calls).
     import numpy as np
     import pytest
@@ -102,7 +102,7 @@ def test_default_algorithm_routes():
     assert isinstance(lsx.default_algorithm(A, b, lsx.OperatorAssumptions(condition="WellConditioned")),
                       lsx.MI355XLUFactorization)
     S = lsx.CSRMatrix(np.arange(5, dtype=np.int32), np.arange(4, dtype=np.int32), np.ones(4), (4, 4))
-    assert isinstance(lsx.default_algorithm(S, b), lsx.MI355XSimpleGMRES)
+    assert isinstance(lsx.default_algorithm(S, b), lsx.MI355XKrylovJL_GMRES)
     assert isinstance(lsx.default_algorithm(lsx.BlockDiagonal([np.eye(2), np.eye(2)]), b),
                       lsx.MI355XBatchedLUFactorization)
     for cond in ("VeryIllConditioned", "SuperIllConditioned"):

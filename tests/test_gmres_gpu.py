@@ -257,3 +257,118 @@ def test_malformed_csr_is_rejected(lsx):
     assert rb["status"] == _abi.LSX_EINVAL
     assert rg["status"] == 0 and rg["retcode"] == lsx.ReturnCode.Success
     srv.close()
+
+
+# ---- KrylovJL_GMRES (src/iterative_wrappers.jl:200-228) on the device GMRES
+
+def _csr(lsx, M):
+    return lsx.CSRMatrix.from_scipy(sp.csr_matrix(M))
+
+
+def test_krylovjl_gmres_reference_interface(lsx):
+    # test/Core/basictests.jl:712-735 ("GMRES" and "GMRES_prec", gmres_restart = 5) through
+    # test_interface (:64-91, real systems prob1 / prob2 of :30-57, cache_kwargs of :62) and
+    # test_tolerance_update (:93-104) on prob5
+    n = 8
+    rng = np.random.default_rng(712)
+    A1, A2 = np.eye(n), np.eye(n) / 2
+    b1, b2 = rng.random(n), rng.random(n)
+    kw = dict(abstol=1e-8, reltol=1e-8, maxiters=30)
+    precs = lambda A, p=None: (1.0 / A.diagonal(), None)  # (Diagonal(inv.(diag(A))), I)
+    for alg in (lsx.MI355XKrylovJL_GMRES(gmres_restart=5),
+                lsx.MI355XKrylovJL_GMRES(gmres_restart=5, precs=precs, ldiv=False)):
+        close = lambda x, y: np.linalg.norm(x - y) <= 1.5e-8 * max(np.linalg.norm(x), np.linalg.norm(y))
+        sol = lsx.solve(lsx.LinearProblem(_csr(lsx, A1), b1.copy()), alg, **kw)
+        assert close(A1 @ sol.u, b1)
+        sol = lsx.solve(lsx.LinearProblem(_csr(lsx, A2), b2.copy()), alg, **kw)
+        assert close(A2 @ sol.u, b2)
+        cache = lsx.init(lsx.LinearProblem(_csr(lsx, A1), b1.copy()), alg, **kw)
+        assert close(A1 @ lsx.solve_(cache).u, b1)
+        cache.A = _csr(lsx, A2)
+        assert close(A2 @ lsx.solve_(cache).u, b1)
+        cache.A = _csr(lsx, A2)
+        cache.b = b2.copy()
+        assert close(A2 @ lsx.solve_(cache).u, b2)
+        # test_tolerance_update: A5 = A5_' A5_, A5_ = I - 0.01 Tridiagonal(ones) + 0.5 e1 e8'
+        A5_ = np.eye(n) - 0.01 * (np.eye(n) + np.eye(n, k=1) + np.eye(n, k=-1))
+        A5_[0, 7] += 0.5
+        A5 = A5_.T @ A5_
+        u5 = np.ones(n)
+        cache = lsx.init(lsx.LinearProblem(_csr(lsx, A5), A5 @ u5), alg)
+        cache.reltol, cache.abstol = 1e-2, 1e-8
+        u1 = lsx.solve_(cache).u.copy()
+        cache.reltol, cache.abstol = 1e-8, 1e-8
+        u2 = lsx.solve_(cache).u
+        assert np.linalg.norm(u2 - u5) < np.linalg.norm(u1 - u5)
+
+
+def test_krylovjl_gmres_reference_issues(lsx):
+    # test/Core/basictests.jl:1112-1119 (issue 347: a 4 x 4 system with default options agrees with
+    # LU) and :1121-1142 (issue 869: the memory keyword sizes the workspace; memory = 30 solves a
+    # sprand(100, 100, 0.1) + 10I system to ||Ax - b|| < 1e-6)
+    rng = np.random.default_rng(347)
+    A = rng.random((4, 4))
+    b = rng.random(4)
+    sol = lsx.solve(lsx.LinearProblem(_csr(lsx, A), b.copy()), lsx.MI355XKrylovJL_GMRES())
+    x = np.linalg.solve(A, b)
+    assert np.linalg.norm(sol.u - x) <= 1.5e-8 * np.linalg.norm(x)
+    S = sp.random(100, 100, density=0.1, random_state=869, format="csr") + 10 * sp.identity(100, format="csr")
+    b = np.random.default_rng(869).random(100)
+    sol = lsx.solve_(lsx.init(lsx.LinearProblem(lsx.CSRMatrix.from_scipy(S), b.copy()),
+                              lsx.MI355XKrylovJL_GMRES(memory=30)))
+    assert sol.retcode == lsx.ReturnCode.Success
+    assert np.linalg.norm(S @ sol.u - b) < 1e-6
+    sol2 = lsx.solve_(lsx.init(lsx.LinearProblem(lsx.CSRMatrix.from_scipy(S), b.copy()),
+                               lsx.MI355XKrylovJL_GMRES(memory=10)))
+    assert isinstance(sol2.u, np.ndarray) and sol2.u.shape == (100,)
+
+
+def test_krylovjl_gmres_resize(lsx):
+    # test/Core/resize.jl:5-48,70-72: solve at n = 3, resize to 6, solve again (atol 1e-6,
+    # maxiters = 100)
+    rng = np.random.default_rng(70)
+    A0 = rng.random((3, 3)) + 5 * np.eye(3)
+    cache = lsx.init(lsx.LinearProblem(_csr(lsx, A0), rng.random(3)), lsx.MI355XKrylovJL_GMRES(), maxiters=100)
+    sol1 = lsx.solve_(cache)
+    assert sol1.retcode == lsx.ReturnCode.Success and len(sol1.u) == 3
+    lsx.resize_(cache, 6)
+    assert cache.isfresh
+    A1 = rng.random((6, 6)) + 5 * np.eye(6)
+    b1 = rng.random(6)
+    expected = np.linalg.solve(A1, b1)
+    cache.A = _csr(lsx, A1)
+    cache.b = b1
+    cache.u = np.zeros(6)
+    sol2 = lsx.solve_(cache)
+    assert sol2.retcode == lsx.ReturnCode.Success and len(sol2.u) == 6
+    np.testing.assert_allclose(sol2.u, expected, atol=1e-6)
+
+
+def test_krylovjl_gmres_options_map_to_the_device_solve(lsx):
+    # gmres_restart = 0: no restart, memory = min(20, n) (src/iterative_wrappers.jl:211-214): the
+    # same solve as SimpleGMRES(restart = false, memory = 20), and the oracle's; gmres_restart = k:
+    # SimpleGMRES(restart = true, memory = k).  Without restart and maxiters above the basis the
+    # device holds (128 vectors at N = 400), the solve restarts every 128 iterations.
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(20, 2.0)
+    N = 400
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    b = np.random.default_rng(3).standard_normal(N)
+    d = np.full(N, 4.0)
+    kw = dict(Pl=d, abstol=0.0, reltol=1e-10)
+    k0 = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XKrylovJL_GMRES(), maxiters=100, **kw)
+    s0 = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XSimpleGMRES(restart=False, memory=20), maxiters=100, **kw)
+    np.testing.assert_array_equal(k0.u, s0.u)
+    assert k0.iters == s0.iters and k0.retcode == s0.retcode
+    ref = O.gmres(rp, ci, v, b, memory=20, restart=False, abstol=0.0, reltol=1e-10, maxiters=100, pl=d)
+    assert ref["iters"] == k0.iters and ref["retcode"] == int(k0.retcode)
+    np.testing.assert_allclose(k0.u, ref["u"], rtol=1e-9, atol=1e-12)
+    k5 = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XKrylovJL_GMRES(gmres_restart=7), maxiters=300, **kw)
+    s5 = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XSimpleGMRES(restart=True, memory=7), maxiters=300, **kw)
+    np.testing.assert_array_equal(k5.u, s5.u)
+    assert k5.iters == s5.iters
+    big = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XKrylovJL_GMRES(), maxiters=400, Pl=d, abstol=0.0,
+                    reltol=1e-13)
+    ref = O.gmres(rp, ci, v, b, memory=128, restart=True, abstol=0.0, reltol=1e-13, maxiters=400, pl=d)
+    assert big.iters == ref["iters"] and int(big.retcode) == ref["retcode"]
+    np.testing.assert_allclose(big.u, ref["u"], rtol=1e-8, atol=1e-12)

@@ -281,7 +281,7 @@ def test_adjoint_fallbacks(lsx):
 
 
 def test_default_solver_routes_to_device_algorithms(lsx):
-    # solve(prob) with no algorithm: dense -> MI355XLUFactorization, CSR -> MI355XSimpleGMRES,
+    # solve(prob) with no algorithm: dense -> MI355XLUFactorization, CSR -> MI355XKrylovJL_GMRES,
     # BlockDiagonal -> MI355XBatchedLUFactorization, each answering to its own bar
     import scipy.sparse as sp
     from linearsolve_jl_amd import problems as P
@@ -295,7 +295,7 @@ def test_default_solver_routes_to_device_algorithms(lsx):
     S = lsx.CSRMatrix(rp, ci, v, (400, 400))
     bs = rng.standard_normal(400)
     sol = lsx.solve(lsx.LinearProblem(S, bs), Pl=np.full(400, 4.0), reltol=1e-10)
-    assert isinstance(sol.cache.alg, lsx.MI355XSimpleGMRES) and sol.retcode == lsx.ReturnCode.Success
+    assert isinstance(sol.cache.alg, lsx.MI355XKrylovJL_GMRES) and sol.retcode == lsx.ReturnCode.Success
     Ssp = sp.csr_matrix((v, ci, rp), shape=(400, 400))
     assert np.linalg.norm(Ssp @ sol.u - bs) <= 1e-8 * np.linalg.norm(bs)
     blocks = [P.dense_block(n, seed=n)[0] for n in (5, 17, 40)]
