@@ -59,7 +59,7 @@ struct BatchArgs {
     const double* b;
     double* x;
     int64_t* ipiv;            // LAPACK swap sequence (1-based), packed like b
-    int32_t* perm;            // logical -> physical rows, packed like b
+    int32_t* perm;            // row i of P A is row perm[i] of A, packed like b
     int64_t* info;
     int32_t* iters;           // refinement steps per system (nullable)
     double* berr;             // FP32 path: final backward error per system (nullable)
