@@ -40,9 +40,19 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int BT = 256;            // threads per system
 constexpr int BNMAX = 512;         // largest system
-constexpr int PBUF = BNMAX * 17;   // panel buffer in doubles: 512 x (16 + 1) or 512 x (8 + 1)
-constexpr int SB = 64;             // row block of the triangular solves (one wave)
-constexpr int RBT = 4, CBT = 2;    // trailing-update block of a wave, in 16 x 16 tiles
+constexpr int RPT = (BNMAX + BT - 1) / BT;  // panel rows per thread
+static_assert(RPT <= 2, "a pivot row is held by the first or the last of a thread's panel rows");
+
+// Per factor type: systems (workgroups) per CU, the trailing-update block of a wave in 16 x 16
+// tiles, whether L21 and the pivot row are held (L21 in LDS, the pivot row in registers) or read
+// where they are used, and the row block SB of the triangular solves.
+// FP64: two systems per CU (up to 256 registers, 80 KB of LDS each) with L21 in LDS and 64-row
+// solve blocks.  FP32: four systems per CU (128 registers, 21 KB of LDS each), L21 read back
+// from the factor in L2 and 32-row solve blocks; the extra systems hide the latency of its
+// refinement passes.  (FP64 at four systems per CU spills registers and runs slower.)
+template <typename T> struct BatchCfg;
+template <> struct BatchCfg<double> { static constexpr int WG = 2, RB = 4, CB = 2, SB = 64; static constexpr bool HOLD = true; };
+template <> struct BatchCfg<float> { static constexpr int WG = 4, RB = 2, CB = 2, SB = 32; static constexpr bool HOLD = false; };
 
 // Panel-width index of a system without a tile table.
 __host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : 1; }
@@ -68,8 +78,15 @@ struct BatchArgs {
     uint32_t* counter;
 };
 
+template <typename T>
 struct BatchShared {
-    double pbuf[PBUF];         // panel (factor type, row stride BW + 1) / staged diagonal blocks
+    static constexpr int SB = BatchCfg<T>::SB;
+    static_assert(BNMAX <= SB * (SB + 1), "the FP32 residual is kept in the solve staging block");
+    double l11[BatchCfg<T>::HOLD ? 1 : 16][17];  // L11 for U12 (when L21 is held: its first rows)
+    union {
+        double l21[BatchCfg<T>::HOLD ? BNMAX * 17 : 1];  // held L21 (row r of the panel at r * 17)
+        double dblk[SB][SB + 1];   // staged diagonal block of the solves / FP32 path: the residual
+    };
     double wkey[BT / 64];
     int wrow[BT / 64];
     double rowbuf[2][16];      // pivot row and the row it displaces (panel columns)
@@ -109,7 +126,8 @@ template <> struct BMma<float> {
 };
 
 // Block-wide max of two non-negative values (all threads get both).
-__device__ __forceinline__ void block_max2(BatchShared& sh, double& a, double& b) {
+template <typename T>
+__device__ __forceinline__ void block_max2(BatchShared<T>& sh, double& a, double& b) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -125,11 +143,11 @@ __device__ __forceinline__ void block_max2(BatchShared& sh, double& a, double& b
     __syncthreads();
 }
 
-// Column C of a panel held in registers (v: thread tid's logical rows tid and tid + BT): pivot
+// Column C of a panel held in registers (v: thread tid's logical rows tid + k BT): pivot
 // search, the interchange of rows C and the pivot row through LDS, elimination below row C.
 // Two barriers per column.  Instantiated for C = 0..BW-1 so that v is indexed by constants.
 template <typename T, int BW, int C>
-__device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m, int j0, int jb, int64_t vo,
+__device__ __forceinline__ void panel_col(BatchShared<T>& sh, T (&v)[RPT][BW], int m, int j0, int jb, int64_t vo,
                                           int64_t* ipiv, int64_t& info) {
     if constexpr (C < BW) {
         const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -138,7 +156,7 @@ __device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m,
             double key = -1.0;
             int krow = 0x7fffffff;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < RPT; ++k) {
                 const int r = tid + k * BT;
                 if (r >= C && r < m) {
                     const double t = fabs((double)v[k][C]);
@@ -162,7 +180,11 @@ __device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m,
             // the pivot row and (when they differ) row C through LDS; row C is thread C's first row
             if (tid == (pr & (BT - 1))) {
 #pragma unroll
-                for (int q = 0; q < BW; ++q) sh.rowbuf[0][q] = (double)(pr < BT ? v[0][q] : v[1][q]);
+                for (int q = 0; q < BW; ++q) {
+                    T x = v[0][q];
+                    if constexpr (RPT > 1) x = (pr >= BT) ? v[RPT - 1][q] : x;
+                    sh.rowbuf[0][q] = (double)x;
+                }
             }
             if (pr != C && tid == C) {
 #pragma unroll
@@ -179,27 +201,33 @@ __device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m,
                 }
             }
             __syncthreads();
-            T prow[BW];
+            // the pivot row: held in registers, or read from LDS where it is used
+            T prh[BW];
 #pragma unroll
-            for (int q = 0; q < BW; ++q) prow[q] = (T)sh.rowbuf[0][q];
+            for (int q = 0; q < BW; ++q) prh[q] = BatchCfg<T>::HOLD ? (T)sh.rowbuf[0][q] : T(0);
+            auto prow = [&](int q) -> T { return BatchCfg<T>::HOLD ? prh[q] : (T)sh.rowbuf[0][q]; };
             if (pr != C) {
                 if (tid == C) {
 #pragma unroll
-                    for (int q = 0; q < BW; ++q) v[0][q] = prow[q];
+                    for (int q = 0; q < BW; ++q) v[0][q] = prow(q);
                 }
                 if (tid == (pr & (BT - 1))) {
 #pragma unroll
                     for (int q = 0; q < BW; ++q) {
                         const T t = (T)sh.rowbuf[1][q];
-                        if (pr < BT) v[0][q] = t;
-                        else v[1][q] = t;
+                        if constexpr (RPT > 1) {
+                            if (pr >= BT) v[RPT - 1][q] = t;
+                            else v[0][q] = t;
+                        } else {
+                            v[0][q] = t;
+                        }
                     }
                 }
             }
-            const T pv = prow[C];
+            const T pv = prow(C);
             if (pv == T(0) && info == 0) info = j0 + C + 1;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
+            for (int k = 0; k < RPT; ++k) {
                 const int r = tid + k * BT;
                 if (r > C && r < m) {
                     T l = v[k][C];
@@ -209,7 +237,7 @@ __device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m,
                     }
                     v[k][C] = l;
 #pragma unroll
-                    for (int q = C + 1; q < BW; ++q) v[k][q] = fma(-l, prow[q], v[k][q]);
+                    for (int q = C + 1; q < BW; ++q) v[k][q] = fma(-l, prow(q), v[k][q]);
                 }
             }
     }
@@ -223,32 +251,33 @@ __device__ __forceinline__ void panel_col(BatchShared& sh, T (&v)[2][BW], int m,
 // one gather at the end, which also leaves the system's permutation (logical row -> original row)
 // in sh.perm.  On return A holds L and U in LAPACK's row order.
 template <typename T, int BW>
-__device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
+__device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    T (*P)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(sh.pbuf);
     for (int j0 = 0; j0 < n; j0 += BW) {
         const int jb = min(BW, n - j0);
         const int m = n - j0;
         // rows j0..n-1 of columns j0..n-1 are in place: the panel's permutation starts as identity
         for (int i = j0 + tid; i < n; i += BT) sh.perm[i] = i;
-        // the panel in registers: thread tid holds logical rows tid and tid + BT (m <= 2 BT)
-        T v[2][BW];
+        // the panel in registers: thread tid holds logical rows tid + k BT, k < RPT (m <= RPT BT)
+        T v[RPT][BW];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < RPT; ++k) {
             const int r = tid + k * BT;
 #pragma unroll
             for (int q = 0; q < BW; ++q) v[k][q] = (r < m && q < jb) ? A[(j0 + q) * n + j0 + r] : T(0);
         }
         panel_col<T, BW, 0>(sh, v, m, j0, jb, vo, ipiv, info);
-        // the factored panel, rows in LAPACK's order: to LDS (L11, L21 for the updates) and to A
+        // the factored panel, rows in LAPACK's order, to A; L11 also to LDS for U12, and L21 to
+        // LDS when it is held (else the trailing update reads it back from A, out of L2)
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < RPT; ++k) {
             const int r = tid + k * BT;
             if (r < m) {
 #pragma unroll
                 for (int q = 0; q < BW; ++q) {
-                    P[r][q] = v[k][q];
                     if (q < jb) A[(j0 + q) * n + j0 + r] = v[k][q];
+                    if constexpr (BatchCfg<T>::HOLD) sh.l21[r * 17 + q] = (double)v[k][q];
+                    else if (k == 0 && r < BW) sh.l11[r][q] = (double)v[k][q];
                 }
             }
         }
@@ -269,35 +298,47 @@ __device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t
             __syncthreads();
             const int nchg = sh.nchg;
             // here jb == BW (only the last panel is narrower, and it has no columns right of it).
-            // Right of the panel, one thread per column: the interchanges (rows j0..c1-1 and the
-            // moved rows below, every source read before any row is written; slots past nchg
-            // repeat the last move), then U12 = L11^{-1} A12.  Offsets within a system fit 32 bits.
+            // Right of the panel, one thread per column: the interchanges, then U12 = L11^{-1} A12.
+            // The new rows j0..c1-1 come from anywhere at or below j0, so they are read first.
+            // A moved row below c1 takes the old content of a row in [j0, c1) (a row below the
+            // panel only ever receives what a swap brings from the panel's own rows), which is
+            // not written until U12 is, so the moves go in chunks once the U12 rows are read
+            // (slots past nchg repeat the last move).  Offsets within a system fit 32 bits.
             for (int col = c1 + tid; col < n; col += BT) {
                 const int co = col * n;
-                T u[BW], w[BW];
+                T u[BW];
 #pragma unroll
                 for (int i = 0; i < BW; ++i) u[i] = A[co + sh.perm[j0 + i]];
-                if (nchg > 0) {
 #pragma unroll
-                    for (int t = 0; t < BW; ++t) w[t] = A[co + sh.perm[sh.chg[min(t, nchg - 1)]]];
+                for (int t0 = 0; t0 < BW; t0 += 8) {
+                    if (t0 < nchg) {
+                        T w[8];
 #pragma unroll
-                    for (int t = 0; t < BW; ++t) A[co + sh.chg[min(t, nchg - 1)]] = w[t];
+                        for (int t = 0; t < 8; ++t) w[t] = A[co + sh.perm[sh.chg[min(t0 + t, nchg - 1)]]];
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) A[co + sh.chg[min(t0 + t, nchg - 1)]] = w[t];
+                    }
                 }
 #pragma unroll
                 for (int kk = 0; kk < BW; ++kk) {
 #pragma unroll
-                    for (int i = kk + 1; i < BW; ++i) u[i] = fma(-P[i][kk], u[kk], u[i]);
+                    for (int i = kk + 1; i < BW; ++i) {
+                        const double l = BatchCfg<T>::HOLD ? sh.l21[i * 17 + kk] : sh.l11[BatchCfg<T>::HOLD ? 0 : i][kk];
+                        u[i] = fma(-(T)l, u[kk], u[i]);
+                    }
                     __builtin_amdgcn_sched_barrier(0);  // L11 read column by column, not all at once
                 }
 #pragma unroll
                 for (int i = 0; i < BW; ++i) A[co + j0 + i] = u[i];
             }
             __syncthreads();
+            constexpr int RBT = BatchCfg<T>::RB, CBT = BatchCfg<T>::CB;
             // A22 -= L21 U12 on MFMA: blocks of RBT x CBT 16 x 16 tiles, waves round-robin
             const int mr = n - c1;
             const int tbr = (mr + 16 * RBT - 1) / (16 * RBT), tbc = (mr + 16 * CBT - 1) / (16 * CBT);
             T* A22 = A + c1 * n + c1;
             const T* U12 = A + c1 * n + j0;
+            const T* L21 = A + j0 * n + c1;
             for (int t = wave; t < tbr * tbc; t += BT / 64) {
                 const int rb = (t % tbr) * (16 * RBT), cb = (t / tbr) * (16 * CBT);
                 int lr[RBT];
@@ -327,7 +368,10 @@ __device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t
                         xv[b2] = (lc < mr) ? -U12[lc * n + kk] : T(0);
                     }
 #pragma unroll
-                    for (int a2 = 0; a2 < RBT; ++a2) yv[a2] = rok[a2] ? P[BW + lr[a2]][kk] : T(0);
+                    for (int a2 = 0; a2 < RBT; ++a2) {
+                        if constexpr (BatchCfg<T>::HOLD) yv[a2] = rok[a2] ? (T)sh.l21[(BW + lr[a2]) * 17 + kk] : T(0);
+                        else yv[a2] = rok[a2] ? L21[kk * n + lr[a2]] : T(0);
+                    }
 #pragma unroll
                     for (int a2 = 0; a2 < RBT; ++a2)
 #pragma unroll
@@ -351,19 +395,26 @@ __device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t
     // p, built from the last panel back (row i of the final order is row S(i) of that order).
     for (int i = tid; i < n; i += BT) sh.perm[i] = i;
     __syncthreads();
-    T* Q = reinterpret_cast<T*>(sh.pbuf);
     bool moved = false;  // some later swap exchanges two different rows
     for (int p = (n - 1) / BW; p >= 0; --p) {
         const int j0 = p * BW, jb = min(BW, n - j0), c1 = j0 + jb, m2 = n - c1;
         if (moved) {
-            for (int idx = tid; idx < m2 * jb; idx += BT) {
-                const int r = idx % m2, c = idx / m2;
-                Q[idx] = A[(int64_t)(j0 + c) * n + sh.perm[c1 + r]];
+            // thread tid moves rows c1 + tid + k BT, k < RPT (m2 < RPT BT), of the panel's columns,
+            // through registers: every source is read before any row is written
+            T g[RPT][BW];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int r = tid + k * BT;
+#pragma unroll
+                for (int c = 0; c < BW; ++c) g[k][c] = (r < m2 && c < jb) ? A[(j0 + c) * n + sh.perm[c1 + r]] : T(0);
             }
             __syncthreads();
-            for (int idx = tid; idx < m2 * jb; idx += BT) {
-                const int r = idx % m2, c = idx / m2;
-                A[(int64_t)(j0 + c) * n + c1 + r] = Q[idx];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int r = tid + k * BT;
+#pragma unroll
+                for (int c = 0; c < BW; ++c)
+                    if (r < m2 && c < jb) A[(j0 + c) * n + c1 + r] = g[k][c];
             }
         }
         // S := tau(j0) o ... o tau(c1 - 1) o S, tau(k) exchanging rows k and piv[k]
@@ -384,9 +435,10 @@ __device__ __forceinline__ void factor_sys(BatchShared& sh, T* A, int n, int64_t
 
 // sh.xs := U^{-1} L^{-1} sh.xs (sh.xs already holds P b), arithmetic in FP64.
 template <typename T>
-__device__ __forceinline__ void solve_sys(BatchShared& sh, const T* A, int n) {
+__device__ __forceinline__ void solve_sys(BatchShared<T>& sh, const T* A, int n) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double (*D)[SB + 1] = reinterpret_cast<double (*)[SB + 1]>(sh.pbuf);
+    constexpr int SB = BatchCfg<T>::SB;
+    double (*D)[SB + 1] = sh.dblk;
     // forward: unit lower L
     for (int k0 = 0; k0 < n; k0 += SB) {
         const int kb = min(SB, n - k0);
@@ -439,8 +491,8 @@ __device__ __forceinline__ void solve_sys(BatchShared& sh, const T* A, int n) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
-    __shared__ BatchShared sh;
+__global__ __launch_bounds__(BT, BatchCfg<T>::WG * BT / 256) void k_batch_lu(BatchArgs a) {  // min waves per SIMD
+    __shared__ BatchShared<T> sh;
     __shared__ int s_item;
     const int tid = threadIdx.x;
     while (true) {
@@ -484,7 +536,7 @@ __global__ __launch_bounds__(BT, 2) void k_batch_lu(BatchArgs a) {
             block_max2(sh, an, bn);
             for (int i = tid; i < n; i += BT) sh.xr[i] = sh.xs[i];
             __syncthreads();
-            double* rr = sh.pbuf;  // residual by original row
+            double* rr = &sh.dblk[0][0];  // residual by original row (n <= (SB + 1) SB)
             int it = 0;
             double be = 0.0;
             while (true) {
@@ -545,7 +597,7 @@ int batch_launch_raw(lsx_ctx* ctx, hipStream_t st, int64_t count, const int32_t*
     a.perm = perm;
     a.info = info;
     a.counter = counter;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)ctx->num_cus * 2));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(count, (int64_t)ctx->num_cus * BatchCfg<double>::WG));
     hipLaunchKernelGGL(k_batch_lu<double>, dim3(grid), dim3(BT), 0, st, a);
     LSX_HIP(ctx, hipGetLastError());
     return LSX_OK;
@@ -571,15 +623,15 @@ static void batch_enqueue(lsx_batch* B) {
     a.maxref = B->maxref;
     a.tol = B->tol;
     a.counter = B->counter;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(B->count, (int64_t)B->ctx->num_cus * 2));
+    auto grid = [&](int wg) { return (int)std::max<int64_t>(1, std::min<int64_t>(B->count, (int64_t)B->ctx->num_cus * wg)); };
     if (B->dtype == LSX_F64) {
         a.A = B->A;
         a.A64 = nullptr;
-        hipLaunchKernelGGL(k_batch_lu<double>, dim3(grid), dim3(BT), 0, B->stream, a);
+        hipLaunchKernelGGL(k_batch_lu<double>, dim3(grid(BatchCfg<double>::WG)), dim3(BT), 0, B->stream, a);
     } else {
         a.A = B->A32;
         a.A64 = B->A;
-        hipLaunchKernelGGL(k_batch_lu<float>, dim3(grid), dim3(BT), 0, B->stream, a);
+        hipLaunchKernelGGL(k_batch_lu<float>, dim3(grid(BatchCfg<float>::WG)), dim3(BT), 0, B->stream, a);
     }
 }
 
