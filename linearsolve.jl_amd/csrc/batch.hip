@@ -14,8 +14,8 @@
 // Per system (BT threads): right-looking blocked LU with panels of BW columns held in LDS.  A
 // panel's row interchanges are applied in memory to the columns right of it as U12 is formed
 // (one thread per column moves the at most 2 BW rows involved), so the trailing update
-// A22 -= L21 U12 reads and writes contiguous rows; it runs on MFMA (FP64 16x16x4 or FP32 16x16x4,
-// 64 x 32 blocks per wave) with L21 from LDS and U12 from L2.  The L columns take the
+// A22 -= L21 U12 reads and writes contiguous rows; it runs on MFMA (FP64 16x16x4 with 64 x 32
+// blocks per wave, L21 from LDS; FP32 16x16x4 with 32 x 32 blocks, L21 from L2) and U12 from L2.  The L columns take the
 // interchanges of later panels in one gather after the last panel (like dlaswp on the left, but
 // once).  The panel width BW (8 or 16) is chosen per system from a size-binned tile
 // table (set by the autotuner through lsx_batch_set_tiles).  Triangular solves go by blocks of 64
