@@ -36,6 +36,7 @@ constexpr int GLEAD = 16;           // from this many workgroups on, all-reduces
 
 struct GShared {
     double wsum[2][GNW];
+    double lsum[2][GNW];        // workgroup 0 from GLEAD workgroups on: wave sums of the partials
     uint32_t rbuf[2][2 * 256];  // 2 words per workgroup, up to 256 workgroups
 };
 
@@ -79,17 +80,32 @@ struct GmresCtx {
             st_relaxed_u64(slot + 2 * w, ll_pack((uint32_t)__double2loint(s), seq));
             st_relaxed_u64(slot + 2 * w + 1, ll_pack((uint32_t)__double2hiint(s), seq));
         }
-        // Small P: every workgroup reads all P partials and sums them.  From GLEAD workgroups on,
-        // workgroup 0 alone reads them, sums them in the same order and publishes the total, so
-        // the others poll two words instead of 2P (the polling traffic grows as P^2 otherwise).
+        // Small P: every workgroup reads all P partials and sums them in order.  From GLEAD
+        // workgroups on, workgroup 0 alone reads them (thread p reads partial p), sums them by a
+        // fixed tree (wave sums, then the waves in order) and publishes the total, so the others
+        // poll two words instead of 2P (the polling traffic grows as P^2 otherwise).
         const bool lead = a.P >= GLEAD;
-        if (!lead || w == 0) {
+        if (!lead) {
             if (tid < 2 * a.P) sh.rbuf[par][tid] = poll(slot + tid, seq);
             __syncthreads();
             double tot = 0.0;
             for (int p = 0; p < a.P; ++p)
                 tot += __hiloint2double((int)sh.rbuf[par][2 * p + 1], (int)sh.rbuf[par][2 * p]);
-            if (lead && tid == 0) {
+            return tot;
+        }
+        if (w == 0) {
+            double pv = 0.0;
+            if (tid < a.P) {
+                const uint32_t lo = poll(slot + 2 * tid, seq), hi = poll(slot + 2 * tid + 1, seq);
+                pv = __hiloint2double((int)hi, (int)lo);
+            }
+            pv = wave_sum(pv);
+            if (lane == 0) sh.lsum[par][wave] = pv;
+            __syncthreads();
+            double tot = 0.0;
+#pragma unroll
+            for (int i = 0; i < GNW; ++i) tot += sh.lsum[par][i];
+            if (tid == 0) {
                 uint64_t* res = a.red + 4 * (int64_t)a.P + 8 + 2 * par;
                 st_relaxed_u64(res, ll_pack((uint32_t)__double2loint(tot), seq));
                 st_relaxed_u64(res + 1, ll_pack((uint32_t)__double2hiint(tot), seq));
@@ -113,6 +129,7 @@ struct GmresCtx {
                     atomicOr((unsigned long long*)&a.out->err, 1ull);
                     break;
                 }
+                __builtin_amdgcn_s_sleep(1);  // back off: many workgroups poll the same words
             }
         }
         return (uint32_t)x;
@@ -137,6 +154,7 @@ struct GmresCtx {
                         atomicOr((unsigned long long*)&a.out->err, 2ull);
                         break;
                     }
+                    __builtin_amdgcn_s_sleep(1);
                 }
             }
             __atomic_thread_fence(__ATOMIC_ACQUIRE);
