@@ -1,8 +1,8 @@
 """ctypes binding of liblsx.so, the gfx950 C-ABI library declared in include/lsx.h.
 
-This is the Python form of the `ccall` layer a Julia package would use (the Julia stubs are in
-INTEGRATION.md).  Loading fails loudly when the library has not been built: there is no CPU
-fallback behind this module.
+This is the Python form of the `ccall` layer a Julia package would use: the same entry points
+with the same argument types.  Loading fails loudly when the library has not been built: there
+is no CPU fallback behind this module.
 """
 import ctypes
 import os
