@@ -117,6 +117,26 @@ def test_fp32_refined_blocks(lsx):
         off += n
 
 
+def test_fp32_refined_pivoting_blocks(lsx):
+    # FP32 factors of systems that pivot in almost every column (iid N(0,1) blocks), sizes on both
+    # sides of 256 rows and of the panel widths: the interchanges applied in memory must leave
+    # factors that refine every block to the target
+    rng = np.random.default_rng(55)
+    sizes = list(rng.integers(1, 513, size=60)) + [1, 7, 8, 9, 15, 16, 17, 255, 256, 257, 511, 512]
+    blocks = [rng.standard_normal((int(s), int(s))) for s in sizes]
+    b = rng.standard_normal(int(sum(sizes)))
+    alg = lsx.MI355XBatchedLUFactorization(precision="fp32", tol=1e-14)
+    sol = lsx.solve(lsx.LinearProblem(lsx.BlockDiagonal(blocks), b), alg)
+    assert sol.retcode == lsx.ReturnCode.Success
+    assert np.all(alg.last_info == 0) and np.all(alg.last_iters <= 10)
+    off = 0
+    for i, Bk in enumerate(blocks):
+        n = Bk.shape[0]
+        assert backward_error(Bk, sol.u[off:off + n], b[off:off + n]) <= 2e-14, f"block {i} n={n}"
+        assert alg.last_backward_error[i] <= 1e-14
+        off += n
+
+
 def test_fp32_singular_block(lsx):
     rng = np.random.default_rng(6)
     blocks = [rng.standard_normal((30, 30)) + 5 * np.eye(30) for _ in range(3)]
