@@ -743,8 +743,8 @@ __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t l
     __shared__ double Wb[2][DB];       // solution of the block being consumed (double-buffered)
     __shared__ double Red[DB];
     __shared__ double Yb[DB];
-    double bin = 0.0;
-    if (tid < DB && r0 + tid < n) bin = In[r0 + tid];
+    __shared__ double Bb[DB];          // the block's right-hand side
+    if (tid < DB) Bb[tid] = (r0 + tid < n) ? In[r0 + tid] : 0.0;
     for (int idx = tid; idx < DB * DB; idx += 256) {
         const int i = idx >> 6, k = idx & (DB - 1);
         Ds[i][k] = (double)Dinv[(int64_t)bw * DB * DB + (int64_t)i * DB + k];
@@ -758,16 +758,8 @@ __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t l
     auto issue = [&](int t) {
         if (tid < 2 * DB) x = ld_relaxed_u64(msg + (size_t)jblk(t) * TMW + tid);
     };
-    if (nj > 0) issue(0);
-    for (int t = 0; t < nj; ++t) {
-        const int j = jblk(t), buf = t & 1;
-        const int64_t srow = (int64_t)j * DB + r;
-        double lv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int64_t ocol = r0 + 16 * g + q;
-            lv[q] = (ocol < n && srow < n) ? (double)F[ocol * ld + srow] : 0.0;
-        }
+    // the solution of chain block j into Wb[buf] (bounded wait)
+    auto receive = [&](int j, int buf) {
         if (tid < 2 * DB && (uint32_t)(x >> 32) != seq) {
             const uint64_t* m = msg + (size_t)j * TMW + tid;
             const int64_t t0 = wall_clock64();
@@ -781,17 +773,29 @@ __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t l
                 }
             }
         }
-        {
-            const uint32_t mine = (uint32_t)x;
-            const uint32_t other = (uint32_t)__shfl_xor((int)mine, 1);
-            if (tid < 2 * DB && !(tid & 1)) Wb[buf][tid >> 1] = __hiloint2double((int)other, (int)mine);
+        const uint32_t mine = (uint32_t)x;
+        const uint32_t other = (uint32_t)__shfl_xor((int)mine, 1);
+        if (tid < 2 * DB && !(tid & 1)) Wb[buf][tid >> 1] = __hiloint2double((int)other, (int)mine);
+    };
+    if (nj > 0) issue(0);
+    // every block but the last one in the chain: wave lanes over the source rows
+    for (int t = 0; t + 1 < nj; ++t) {
+        const int j = jblk(t), buf = t & 1;
+        const int64_t srow = (int64_t)j * DB + r;
+        double lv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int64_t ocol = r0 + 16 * g + q;
+            lv[q] = (ocol < n && srow < n) ? (double)F[ocol * ld + srow] : 0.0;
         }
-        if (t + 1 < nj) issue(t + 1);
+        receive(j, buf);
+        issue(t + 1);
         __syncthreads();
         const double wr = Wb[buf][r];
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc[q] = fma(lv[q], wr, acc[q]);
     }
+    // their sums across the wave, while the last block's solution travels
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         double v = acc[q];
@@ -799,18 +803,43 @@ __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t l
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (r == 0) Red[16 * g + q] = v;
     }
+    // the last block: four threads per output row (i = tid / 4), each with 16 values of column
+    // r0 + i of F, loaded before the wait, and a two-step sum over the four
+    double last = 0.0;
+    if (nj > 0) {
+        const int jl = jblk(nj - 1), buf = (nj - 1) & 1;
+        const int i = tid >> 2, p4 = tid & 3;
+        double f[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int64_t srow = (int64_t)jl * DB + 16 * p4 + kk;
+            f[kk] = (r0 + i < n && srow < n) ? (double)F[(r0 + i) * ld + srow] : 0.0;
+        }
+        receive(jl, buf);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) last = fma(f[kk], Wb[buf][16 * p4 + kk], last);
+        last += __shfl_xor(last, 1);
+        last += __shfl_xor(last, 2);
+    }
     __syncthreads();
-    if (tid < DB) Yb[tid] = bin - Red[tid];
+    if ((tid & 3) == 0) Yb[tid >> 2] = Bb[tid >> 2] - Red[tid >> 2] - last;
     __syncthreads();
-    if (tid < DB) {
+    // y = Dinv^T Yb, again four threads per row, 16 terms each
+    {
+        const int i = tid >> 2, p4 = tid & 3;
         double y = 0.0;
-#pragma unroll 8
-        for (int k = 0; k < DB; ++k) y = fma(Ds[tid][k], Yb[k], y);
-        const int64_t row = r0 + tid;
-        if (row < n) Out[FIRST ? row : (int64_t)perm[row]] = y;
-        uint64_t* mine = msg + (size_t)bw * TMW;
-        st_relaxed_u64(mine + 2 * tid, ll_pack((uint32_t)__double2loint(y), seq));
-        st_relaxed_u64(mine + 2 * tid + 1, ll_pack((uint32_t)__double2hiint(y), seq));
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) y = fma(Ds[i][16 * p4 + kk], Yb[16 * p4 + kk], y);
+        y += __shfl_xor(y, 1);
+        y += __shfl_xor(y, 2);
+        if (p4 == 0) {
+            const int64_t row = r0 + i;
+            if (row < n) Out[FIRST ? row : (int64_t)perm[row]] = y;
+            uint64_t* mine = msg + (size_t)bw * TMW;
+            st_relaxed_u64(mine + 2 * i, ll_pack((uint32_t)__double2loint(y), seq));
+            st_relaxed_u64(mine + 2 * i + 1, ll_pack((uint32_t)__double2hiint(y), seq));
+        }
     }
 }
 
