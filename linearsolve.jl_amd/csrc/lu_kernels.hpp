@@ -725,10 +725,12 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
 //   FIRST:  Out = U^{-T} In         (U^T is lower triangular: blocks in forward order)
 //   !FIRST: Out[perm[i]] = (L^{-T} In)[i]   (L^T unit upper: blocks in backward order)
 // The tile of block row b and block column j of the transposed factor is the (j, b) tile of F
-// read by columns: lane r of wave g reads F[(r0 + 16 g + q) ld + j DB + r] (coalesced), keeps
-// its own partial sums of the 16 output rows 16 g + q over all earlier blocks, and the sums are
-// reduced across the wave once at the end.  The diagonal block's inverse is the transpose of the
-// stored one.  Block solutions travel as in k_trsv (two 64-bit words per value, 32-bit halves
+// read by columns: lane r of wave g reads F[(r0 + 16 g + q) ld + j DB + r] (coalesced) and keeps
+// its own partial sums of the 16 output rows 16 g + q over all earlier blocks but the last one;
+// those are reduced across the wave while the last block's solution travels.  The last block's
+// tile is read by rows instead (four threads per output row, 16 values each, loaded before the
+// wait), so after its solution arrives only a two-step shuffle remains; the product with the
+// diagonal block's inverse (the transpose of the stored one) is split the same way.  Block solutions travel as in k_trsv (two 64-bit words per value, 32-bit halves
 // each with the launch's sequence number).
 template <typename T, bool FIRST>
 __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t ld, int64_t n,
