@@ -372,3 +372,30 @@ def test_krylovjl_gmres_options_map_to_the_device_solve(lsx):
     ref = O.gmres(rp, ci, v, b, memory=128, restart=True, abstol=0.0, reltol=1e-13, maxiters=400, pl=d)
     assert big.iters == ref["iters"] and int(big.retcode) == ref["retcode"]
     np.testing.assert_allclose(big.u, ref["u"], rtol=1e-8, atol=1e-12)
+
+
+def test_max_memory_bounds_the_unrestarted_basis(lsx):
+    # lsx_gmres_max_memory: 128 vectors for small N, fewer as each workgroup's rows grow; a solve
+    # without restart at that bound runs, one vector more is refused with LSX_EINVAL
+    import ctypes
+    from linearsolve_jl_amd import _abi
+    P = problems(lsx)
+    L = _abi.lib()
+    prev = 129
+    for m in (8, 22, 45, 90, 200):
+        rp, ci, v = P.convection_diffusion(m, 1.0)
+        N = m * m
+        h = lsx._CSRHandle(lsx.Context.get(0), lsx.CSRMatrix(rp, ci, v, (N, N)))
+        lim = ctypes.c_int32()
+        assert L.lsx_gmres_max_memory(h.h, ctypes.byref(lim)) == 0
+        assert 1 <= lim.value <= prev and (N > 512 or lim.value == 128), (m, lim.value)
+        prev = lim.value
+        b = np.ones(N)
+        x = np.empty(N)
+        st = _abi.GmresStats()
+        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-12, lim.value,
+                               None, None, ctypes.byref(st))
+        assert rc == 0 and st.iters <= lim.value
+        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-12, lim.value + 1,
+                               None, None, ctypes.byref(st))
+        assert rc == _abi.LSX_EINVAL
