@@ -60,6 +60,13 @@ int lsx_ctx_create(int device, lsx_ctx** out);
 int lsx_ctx_destroy(lsx_ctx* ctx);
 const char* lsx_last_error(const lsx_ctx* ctx);
 int lsx_version(void);
+/* Bound, in seconds, of every device-side wait of later calls on this context: the panel
+ * exchange of getrf, the block chain of the triangular solves and the all-reduces of GMRES.
+ * 0 restores the defaults (5 s for the LU waits, 20 s for GMRES).  When a wait reaches its bound
+ * the call returns LSX_ETIMEOUT; every other wait of the same call then gives up at once, so the
+ * call ends promptly.  No reference counterpart (LAPACK and SparseArrays never wait on other
+ * processors); for callers that must bound a call's time. */
+int lsx_ctx_set_wait_bound(lsx_ctx* ctx, double seconds);
 /* Device memory helpers for hosts without a GPU array package (device pointers). */
 int lsx_malloc(lsx_ctx* ctx, size_t bytes, void** dptr);
 int lsx_free(lsx_ctx* ctx, void* dptr);
@@ -94,6 +101,15 @@ int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double*
  * src/adjoint_factorization.jl:162-176).  FP64 handles only.  B and X may alias. */
 int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double* X, int64_t ldx);
 
+/* norms[0] = ||B - A X||_2 and norms[1] = ||B||_2 (2-norms over all nrhs columns), computed in
+ * FP64 on the device.  A is the matrix before factorization (host or device; a host A is staged
+ * into a device copy the handle keeps).  Replaces the GEMV of the post-solve residual check
+ * _check_residual_safety (src/factorization.jl:127-156: mul!(buf, A, y); axpy!(-1, b, buf);
+ * norm(buf) > abstol + reltol * norm(b) -> ReturnCode.APosterioriSafetyFailure), run when
+ * residualsafety = true and the solve refactorized (:646). */
+int lsx_lu_residual_norm(lsx_lu* lu, const double* A, int64_t lda, int64_t nrhs, const double* B, int64_t ldb,
+                         const double* X, int64_t ldx, double* norms);
+
 /* Per-kernel-class device timing of a handle (HIP events around each launch while enabled).
  * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update (128 x 128 tiles),
  * 3 permutation lists, 4 triangular solves, 5 refinement residual/update, 6 low-precision
@@ -123,7 +139,8 @@ typedef struct {
 } lsx_refine_stats;
 
 /* Mixed-precision solve of the factored lu (LSX_F32/LSX_BF16) with FP64 refinement to a normwise
- * backward error <= tol.  mode 0 = IR, 1 = GMRES-IR, 2 = IR, then GMRES-IR if IR stalls, then
+ * backward error <= tol.  Returns 0, or LAPACK-style info > 0 when a refactorization in FP32
+ * (mode 2, below) finds an exactly-zero pivot (no solution then), or < 0 on error.  mode 0 = IR, 1 = GMRES-IR, 2 = IR, then GMRES-IR if IR stalls, then
  * (BF16 factors only) a refactorization in FP32 if GMRES-IR stalls too.  A (host or device) is the original FP64 matrix when it is not already held by the
  * handle (pass NULL after lsx_lu_factor: the handle keeps its FP64 copy).  No reference
  * counterpart does refinement: the API shape follows OpenBLAS32MixedLUFactorization

@@ -32,12 +32,14 @@ __all__ = [
 
 
 class ReturnCode(enum.IntEnum):
-    """Subset of SciMLBase.ReturnCode used by the hot paths (values match lsx_retcode)."""
+    """Subset of SciMLBase.ReturnCode used by the hot paths (values 0-4 match lsx_retcode; the
+    post-solve residual check of src/factorization.jl:127-156 adds APosterioriSafetyFailure)."""
     Default = 0
     Success = 1
     MaxIters = 2
     Infeasible = 3
     Failure = 4
+    APosterioriSafetyFailure = 5
 
 
 def successful_retcode(rc):
@@ -343,13 +345,24 @@ class _LUHandle:
         return u
 
     def refine(self, b, u, tol, max_iters, mode):
+        """(info, stats): info > 0 when a refactorization in FP32 met an exactly-zero pivot."""
         keep, p = _colmajor(b)
         nr = _ncols(b)
         st = _abi.RefineStats()
         up = u.data_ptr() if _is_torch(u) else _ptr_out(u)
-        self.ctx.check(_abi.lib().lsx_lu_refine(self.h, nr, p, self.n, up, self.n, tol, max_iters, mode,
-                                                ctypes.byref(st)), "lsx_lu_refine")
-        return st
+        info = self.ctx.check(_abi.lib().lsx_lu_refine(self.h, nr, p, self.n, up, self.n, tol, max_iters, mode,
+                                                       ctypes.byref(st)), "lsx_lu_refine")
+        return info, st
+
+    def residual_norm(self, A, b, u):
+        """(||b - A u||_2, ||b||_2) in FP64 on the device (norms over all columns)."""
+        ka, pa = _colmajor(A)
+        kb, pb = _colmajor(b)
+        ku, pu = _colmajor(u)
+        out = (ctypes.c_double * 2)()
+        self.ctx.check(_abi.lib().lsx_lu_residual_norm(self.h, pa, self.n, _ncols(b), pb, self.n, pu, self.n, out),
+                       "lsx_lu_residual_norm")
+        return out[0], out[1]
 
 
 class _DenseAlg:
@@ -384,6 +397,8 @@ class MI355XLUFactorization(_DenseAlg):
         n = A.shape[0]
         if A.shape[0] != A.shape[1]:
             raise ValueError("MI355XLUFactorization needs a square A")
+        # src/factorization.jl:646: the residual check runs on solves that (re)factorize
+        check_safety = self.residualsafety and cache.isfresh
         if cache.isfresh:
             h = self._fresh_handle(cache, n)
             info = h.factor(A)
@@ -392,11 +407,12 @@ class MI355XLUFactorization(_DenseAlg):
                 return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
             object.__setattr__(cache, "isfresh", False)
         cache.cacheval.solve(cache.b, cache.u)
-        if self.residualsafety:
-            # src/factorization.jl:127-156
-            r = _np(cache.A) @ _np(cache.u) - _np(cache.b)
-            if np.linalg.norm(r) > cache.abstol + cache.reltol * np.linalg.norm(_np(cache.b)):
-                return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
+        if check_safety:
+            # src/factorization.jl:127-156: ||A y - b|| > abstol + reltol ||b|| (2-norms, the A
+            # of the cache, which the device factorization leaves untouched), on the device
+            rn, bn = cache.cacheval.residual_norm(cache.A, cache.b, cache.u)
+            if rn > cache.abstol + cache.reltol * bn:
+                return LinearSolution(cache.u, None, ReturnCode.APosterioriSafetyFailure, 0, cache)
         return LinearSolution(cache.u, None, ReturnCode.Success, 0, cache)
 
 
@@ -424,8 +440,11 @@ class MI355XMixedLUFactorization(_DenseAlg):
             if info > 0:
                 return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
             object.__setattr__(cache, "isfresh", False)
-        st = cache.cacheval.refine(cache.b, cache.u, self.tol, self.max_iters, self.mode)
+        info, st = cache.cacheval.refine(cache.b, cache.u, self.tol, self.max_iters, self.mode)
         self.last_stats = st
+        if info > 0:
+            # the FP32 refactorization of an escalated BF16 solve is singular: as a zero pivot
+            return LinearSolution(cache.u, None, ReturnCode.Failure, 0, cache)
         rc = ReturnCode.Success if st.converged else ReturnCode.MaxIters
         return LinearSolution(cache.u, st.backward_error, rc, st.iterations, cache,
                               stats=dict(mode_used=st.mode_used, iterations=st.iterations, outer=st.outer,
@@ -899,7 +918,10 @@ class Server:
         self.ctx.check(rc, "lsx_serve_poll")
         keep = self._keep.pop(ticket)
         x = keep[2]
-        return dict(x=x, status=r.status, retcode=ReturnCode(r.retcode), info=r.info, iters=r.iters,
+        # a request that ended in an error (status < 0: a device-side wait gave up, a malformed
+        # CSR, ...) has no valid result: its retcode is Failure whatever the solver reported
+        retcode = ReturnCode(r.retcode) if r.status == _abi.LSX_OK else ReturnCode.Failure
+        return dict(x=x, status=r.status, retcode=retcode, info=r.info, iters=r.iters,
                     resid=r.resid, t_submit=r.t_submit, t_start=r.t_start, t_done=r.t_done)
 
     def drain(self):

@@ -75,6 +75,7 @@ SIGNATURES = [
     ("lsx_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
     ("lsx_ctx_destroy", ctypes.c_int, [_P]),
     ("lsx_last_error", ctypes.c_char_p, [_P]),
+    ("lsx_ctx_set_wait_bound", ctypes.c_int, [_P, _D]),
     ("lsx_malloc", ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     ("lsx_free", ctypes.c_int, [_P, _P]),
     ("lsx_memcpy", ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
@@ -85,6 +86,7 @@ SIGNATURES = [
     ("lsx_lu_get_factors", ctypes.c_int, [_P, _P, _I64, _P]),
     ("lsx_lu_solve", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64]),
     ("lsx_lu_solve_adjoint", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64]),
+    ("lsx_lu_residual_norm", ctypes.c_int, [_P, _P, _I64, _I64, _P, _I64, _P, _I64, _P]),
     ("lsx_lu_profile", ctypes.c_int, [_P, _I32, ctypes.POINTER(Profile)]),
     ("lsx_lu_refine", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _I32, _I32, ctypes.POINTER(RefineStats)]),
     ("lsx_batch_create", ctypes.c_int, [_P, _I64, _P, ctypes.c_int, ctypes.POINTER(_P)]),
@@ -166,3 +168,7 @@ class Context:
             msg = lib().lsx_last_error(self.handle)
             raise LSXError(rc, f"{what}: {msg.decode() if msg else ''}")
         return rc
+
+    def set_wait_bound(self, seconds):
+        """Bound (s) of the device-side waits of later calls; 0 restores the defaults."""
+        self.check(lib().lsx_ctx_set_wait_bound(self.handle, float(seconds)), "lsx_ctx_set_wait_bound")

@@ -1,9 +1,11 @@
 // api.hip — the C-ABI entry points of liblsx.so (declared in include/lsx.h).
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <vector>
 
 #include "lu.hpp"
+#include "residual.hpp"
 
 namespace lsx {
 int fail(lsx_ctx* ctx, int code, const char* fmt, ...) {
@@ -107,6 +109,9 @@ void lu64_free(lsx_lu* lu) {
     (void)hipFree(lu->DU);
     (void)hipFree(lu->wb);
     (void)hipFree(lu->wy);
+    (void)hipFree(lu->astage);
+    (void)hipFree(lu->rpart);
+    if (lu->hrpart) (void)hipHostFree(lu->hrpart);
     if (lu->stream2) (void)hipStreamSynchronize(lu->stream2);
     if (lu->stream_upd) (void)hipStreamSynchronize(lu->stream_upd);
     if (lu->ev_start) (void)hipEventDestroy(lu->ev_start);
@@ -149,6 +154,12 @@ int lsx_ctx_destroy(lsx_ctx* ctx) {
 }
 
 const char* lsx_last_error(const lsx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int lsx_ctx_set_wait_bound(lsx_ctx* ctx, double seconds) {
+    if (!ctx || !(seconds >= 0.0) || seconds > 3600.0) return LSX_EINVAL;
+    ctx->wait_bound_s = seconds;
+    return LSX_OK;
+}
 
 int lsx_malloc(lsx_ctx* ctx, size_t bytes, void** dptr) {
     if (!ctx || !dptr) return LSX_EINVAL;
@@ -258,6 +269,7 @@ int lsx_lu_solve(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb, double*
     int rc = lu_ensure_work(lu, nrhs);
     if (rc) return rc;
     const int64_t n = lu->n;
+    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
     LSX_HIP(ctx, hipMemcpy2DAsync(lu->wb, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                                   hipMemcpyDefault, lu->stream));
     rc = (lu->dtype == LSX_F64) ? lu64_getrs_dev(lu, nrhs, lu->wb, n, lu->wb, n) : mixed_getrs_dev(lu, nrhs, lu->wb, n, lu->wb, n);
@@ -283,6 +295,7 @@ int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb,
     int rc = lu_ensure_work(lu, nrhs);
     if (rc) return rc;
     const int64_t n = lu->n;
+    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
     LSX_HIP(ctx, hipMemcpy2DAsync(lu->wb, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                                   hipMemcpyDefault, lu->stream));
     rc = lu64_getrs_t_dev(lu, nrhs, lu->wb, n, lu->wb, n);
@@ -293,6 +306,62 @@ int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb,
     LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
     lu->prof.collect();
     if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_solve_adjoint: a block wait exceeded its bound");
+    return LSX_OK;
+}
+
+int lsx_lu_residual_norm(lsx_lu* lu, const double* A, int64_t lda, int64_t nrhs, const double* B, int64_t ldb,
+                         const double* X, int64_t ldx, double* norms) {
+    if (!lu || !A || !B || !X || !norms || nrhs < 0) return LSX_EINVAL;
+    lsx_ctx* ctx = lu->ctx;
+    const int64_t n = lu->n;
+    if (lda < n || ldb < n || ldx < n) return fail(ctx, LSX_EINVAL, "lsx_lu_residual_norm: lda/ldb/ldx < n");
+    norms[0] = norms[1] = 0.0;
+    if (nrhs == 0) return LSX_OK;
+    DeviceGuard g(ctx);
+    int rc = lu_ensure_work(lu, nrhs);
+    if (rc) return rc;
+    const hipStream_t st = lu->stream;
+    // A is read where it lies when it is memory of this device, else staged once
+    const double* dA = A;
+    int64_t ldA = lda;
+    if (!is_device_ptr(ctx->device, A)) {
+        if (!lu->astage && hipMalloc(&lu->astage, sizeof(double) * std::max(lu->n, lu->cap) * std::max(lu->n, lu->cap)) != hipSuccess) {
+            lu->astage = nullptr;
+            return fail(ctx, LSX_ENOMEM, "lsx_lu_residual_norm: cannot stage A");
+        }
+        LSX_HIP(ctx, hipMemcpy2DAsync(lu->astage, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                                      hipMemcpyDefault, st));
+        dA = lu->astage;
+        ldA = n;
+    }
+    if (!lu->rpart) {
+        const size_t words = (size_t)resid_part_words(std::max(lu->n, lu->cap));
+        if (hipMalloc(&lu->rpart, sizeof(double) * words) != hipSuccess ||
+            hipHostMalloc(&lu->hrpart, sizeof(double) * words, hipHostMallocDefault) != hipSuccess)
+            return fail(ctx, LSX_ENOMEM, "lsx_lu_residual_norm: cannot allocate the statistics buffers");
+    }
+    // B into wb (the residual overwrites it), X into wy
+    LSX_HIP(ctx, hipMemcpy2DAsync(lu->wb, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, st));
+    LSX_HIP(ctx, hipMemcpy2DAsync(lu->wy, sizeof(double) * n, X, sizeof(double) * ldx, sizeof(double) * n, nrhs,
+                                  hipMemcpyDefault, st));
+    double ssr = 0.0, ssb = 0.0;
+    for (int64_t c0 = 0; c0 < nrhs; c0 += RNR) {
+        const int nr = (int)std::min<int64_t>(RNR, nrhs - c0);
+        resid_launch(st, dA, ldA, n, lu->wy + c0 * n, n, nr, lu->wb + c0 * n, n, lu->wb + c0 * n, n, lu->rpart,
+                     RES_RESIDUAL);
+        LSX_HIP(ctx, hipGetLastError());
+        LSX_HIP(ctx, hipMemcpyAsync(lu->hrpart, lu->rpart, sizeof(double) * resid_part_words(n), hipMemcpyDeviceToHost, st));
+        LSX_HIP(ctx, hipStreamSynchronize(st));
+        double stv[RNR][RSTAT];
+        resid_combine(lu->hrpart, n, nr, stv);
+        for (int s = 0; s < nr; ++s) {
+            ssr += stv[s][1];
+            ssb += stv[s][4];
+        }
+    }
+    norms[0] = std::sqrt(ssr);
+    norms[1] = std::sqrt(ssb);
     return LSX_OK;
 }
 

@@ -15,6 +15,7 @@ struct lsx_ctx {
     int device = 0;
     int num_cus = 0;
     int64_t wall_khz = 100000;  // device wall clock rate (kHz), for bounded device-side waits
+    double wait_bound_s = 0.0;  // > 0: bound of every later device-side wait (seconds); 0: defaults
     std::mutex mu;
     std::string err;
 };
@@ -47,6 +48,14 @@ struct DeviceGuard {
     } while (0)
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Device wall-clock ticks of a device-side wait bound: the context's bound when one is set,
+// else dflt_s seconds.
+inline int64_t wait_ticks(const lsx_ctx* c, double dflt_s) {
+    const double s = c->wait_bound_s > 0.0 ? c->wait_bound_s : dflt_s;
+    const double t = s * 1e3 * (double)c->wall_khz;
+    return t < 1.0 ? 1 : (int64_t)t;
+}
 inline int64_t rup(int64_t a, int64_t b) { return cdiv(a, b) * b; }
 
 // Leading dimension of resident matrices: a multiple of 32 elements, nudged off multiples of
@@ -55,6 +64,16 @@ inline int64_t pad_ld(int64_t n) {
     int64_t ld = rup(n < 1 ? 1 : n, 32);
     if (ld % 512 == 0) ld += 32;
     return ld;
+}
+
+// Whether p is memory of this device (host memory, pinned or pageable, is not).
+inline bool is_device_ptr(int device, const void* p) {
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice && at.device == device;
 }
 
 // Device-side helpers ---------------------------------------------------------------------

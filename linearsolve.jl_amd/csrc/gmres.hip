@@ -38,6 +38,7 @@ struct GShared {
     double wsum[2][GNW];
     double lsum[2][GNW];        // workgroup 0 from GLEAD workgroups on: wave sums of the partials
     uint32_t rbuf[2][2 * 256];  // 2 words per workgroup, up to 256 workgroups
+    int abort;                  // a wait of this solve gave up: leave the solve
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -53,6 +54,8 @@ struct GmresCtx {
     int tid, lane, wave, w;
     int64_t row0;     // first row of this workgroup
     uint32_t nred = 0, nbar = 0;
+    bool dead = false;  // a wait of this solve gave up (uniform in the workgroup): its results
+                        // are invalid and every loop of the solve ends
 
     __device__ GmresCtx(const GmresArgs& a_, GShared& sh_, int w_) : a(a_), sh(sh_) {
         tid = threadIdx.x;
@@ -88,6 +91,7 @@ struct GmresCtx {
         if (!lead) {
             if (tid < 2 * a.P) sh.rbuf[par][tid] = poll(slot + tid, seq);
             __syncthreads();
+            dead = sh.abort != 0;
             double tot = 0.0;
             for (int p = 0; p < a.P; ++p)
                 tot += __hiloint2double((int)sh.rbuf[par][2 * p + 1], (int)sh.rbuf[par][2 * p]);
@@ -102,6 +106,7 @@ struct GmresCtx {
             pv = wave_sum(pv);
             if (lane == 0) sh.lsum[par][wave] = pv;
             __syncthreads();
+            dead = sh.abort != 0;
             double tot = 0.0;
 #pragma unroll
             for (int i = 0; i < GNW; ++i) tot += sh.lsum[par][i];
@@ -114,10 +119,13 @@ struct GmresCtx {
         }
         if (tid < 2) sh.rbuf[par][tid] = poll(a.red + 4 * (int64_t)a.P + 8 + 2 * par + tid, seq);
         __syncthreads();
+        dead = sh.abort != 0;
         return __hiloint2double((int)sh.rbuf[par][1], (int)sh.rbuf[par][0]);
     }
 
-    // Payload of a flag-in-word message once it carries sequence number seq (bounded wait).
+    // Payload of a flag-in-word message once it carries sequence number seq.  The wait ends at
+    // the bound, or at once when any wait of this solve has given up (the solve's error word is
+    // set); either way the workgroup's abort flag is raised, read after the caller's barrier.
     __device__ uint32_t poll(const uint64_t* word, uint32_t seq) const {
         uint64_t x = ld_relaxed_u64(word);
         if ((uint32_t)(x >> 32) != seq) {
@@ -127,6 +135,11 @@ struct GmresCtx {
                 if ((uint32_t)(x >> 32) == seq) break;
                 if (wall_clock64() - t0 > a.timeout) {
                     atomicOr((unsigned long long*)&a.out->err, 1ull);
+                    sh.abort = 1;
+                    break;
+                }
+                if (ld_relaxed_u64(&a.out->err) != 0) {
+                    sh.abort = 1;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);  // back off: many workgroups poll the same words
@@ -152,6 +165,11 @@ struct GmresCtx {
                 while (__hip_atomic_load(a.bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
                     if (wall_clock64() - t0 > a.timeout) {
                         atomicOr((unsigned long long*)&a.out->err, 2ull);
+                        sh.abort = 1;
+                        break;
+                    }
+                    if (ld_relaxed_u64(&a.out->err) != 0) {
+                        sh.abort = 1;
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -160,6 +178,7 @@ struct GmresCtx {
             __atomic_thread_fence(__ATOMIC_ACQUIRE);
         }
         __syncthreads();
+        dead = sh.abort != 0;
     }
 
     // y_r = (A (v ./ pr))_r, terms summed in ascending column order without fusion (the order
@@ -214,6 +233,7 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
     double* vs = qs + RPT * GBS;              // RPT x GBS
     __shared__ int s_flags[4];
     GmresCtx<RPT> g(a, sh, w);
+    if (g.tid == 0) sh.abort = 0;  // ordered before any poll by the first all-reduce's barrier
     const int64_t N = a.N;
     const int tid = g.tid;
     double* x = a.x;
@@ -238,7 +258,9 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
     double rNorm = beta0;
     int64_t iter = 0;
     int status = LSX_RET_DEFAULT;
-    if (beta0 == 0.0) {
+    if (g.dead) {
+        // a wait gave up: the error word tells the host, nothing else is computed
+    } else if (beta0 == 0.0) {
         status = LSX_RET_SUCCESS;
     } else {
         int npass = 0;
@@ -247,7 +269,7 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
         bool solved = rNorm <= eps_;
         int64_t inner_maxiters = a.maxiters;
         bool tired = iter >= a.maxiters;
-        while (!(solved || tired || breakdown)) {
+        while (!(solved || tired || breakdown || g.dead)) {
             int64_t nr = 0;
             if (a.restart) {
 #pragma unroll 2
@@ -285,7 +307,7 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
             ++npass;
             int k = 0;
             bool inner_tired = false;
-            while (!(solved || inner_tired || breakdown)) {
+            while (!(solved || inner_tired || breakdown || g.dead)) {
                 ++k;
                 const double* vcur = a.V + (int64_t)(k - 1) * N;
                 // q = Pl \ (A (Pr \ v_k))
@@ -331,6 +353,7 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
                         }
                     }
                     const double red = g.allreduce(part);
+                    if (g.dead) break;
                     if (ii <= k) {
                         h = red;
                         if (tid == 0) R[nr + ii - 1] = h;
@@ -592,7 +615,7 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     g.red = A->red;
     g.bar = reinterpret_cast<uint32_t*>(A->red + 4 * (int64_t)P);
     g.out = A->out;
-    g.timeout = ctx->wall_khz * 20000;  // 20 s
+    g.timeout = wait_ticks(ctx, 20.0);
     const size_t dynlds = plan.dynlds;
     switch (rpt) {
         case 1: hipLaunchKernelGGL(k_gmres<1>, dim3(P), dim3(GBS), dynlds, st, g); break;
@@ -881,9 +904,10 @@ int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, doub
     if (pr_diag) LSX_HIP(ctx, hipMemcpyAsync(A->dpr, pr_diag, vb, hipMemcpyDefault, st));
     // the launch table, rebuilt when what it holds changes; its tolerance is written on the device
     const int64_t memv = std::max<int64_t>(1, std::min<int64_t>(memory, maxiters));
-    const int64_t key[8] = {blocksize, (pl_diag ? 1 : 0) | (pr_diag ? 2 : 0), memv, restart ? 1 : 0, maxiters,
-                            (int64_t)(uintptr_t)A->ws, (int64_t)(uintptr_t)A->bred, plan.cap};
-    if (!std::equal(key, key + 8, A->bkey)) {
+    const int64_t wt = wait_ticks(ctx, 20.0);
+    const int64_t key[9] = {blocksize, (pl_diag ? 1 : 0) | (pr_diag ? 2 : 0), memv, restart ? 1 : 0, maxiters,
+                            (int64_t)(uintptr_t)A->ws, (int64_t)(uintptr_t)A->bred, plan.cap, wt};
+    if (!std::equal(key, key + 9, A->bkey)) {
         std::vector<GmresArgs> items(nb);
         std::vector<int> first(nb + 1);
         for (int64_t i = 0; i < nb; ++i) {
@@ -909,14 +933,14 @@ int lsx_gmres_solve_batched(lsx_csr* A, int64_t blocksize, const double* b, doub
             a.red = A->bred + i * rw;
             a.bar = reinterpret_cast<uint32_t*>(a.red + 4);
             a.out = A->bout + i;
-            a.timeout = ctx->wall_khz * 20000;
+            a.timeout = wt;
             first[i] = (int)i;
         }
         first[nb] = (int)nb;
         LSX_HIP(ctx, hipMemcpyAsync(A->bitems, items.data(), sizeof(GmresArgs) * nb, hipMemcpyHostToDevice, st));
         LSX_HIP(ctx, hipMemcpyAsync(A->bfirst, first.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice, st));
         LSX_HIP(ctx, hipStreamSynchronize(st));  // the host tables are freed on return
-        std::copy(key, key + 8, A->bkey);
+        std::copy(key, key + 9, A->bkey);
     }
     // the stopping tolerance of the whole batch: eps = abstol + reltol ||Pl \ b||_2
     // (src/simplegmres.jl:110-129: beta is the norm of the whole vector)

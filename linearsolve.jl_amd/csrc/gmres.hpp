@@ -87,7 +87,7 @@ struct lsx_csr {
     int* bflag = nullptr;        // off-block entry found
     int64_t bcap = 0;            // blocks the batched buffers hold
     double* bpart = nullptr;     // partial sums of the stopping norm
-    int64_t bkey[8] = {};        // parameters the device launch table was built for
+    int64_t bkey[9] = {};        // parameters the device launch table was built for
 };
 
 namespace lsx {

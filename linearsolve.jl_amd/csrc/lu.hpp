@@ -50,6 +50,10 @@ struct lsx_lu {
     double* wb = nullptr;
     double* wy = nullptr;
     int64_t wcap = 0;  // columns of wb/wy
+    // residual check: staging copy of a host A, statistics of the residual kernel
+    double* astage = nullptr;
+    double* rpart = nullptr;
+    double* hrpart = nullptr;     // pinned
     // mixed precision state
     float* F32 = nullptr;         // low-precision working factors (FP32 storage)
     int64_t ld32 = 0;
@@ -61,6 +65,11 @@ struct lsx_lu {
 };
 
 namespace lsx {
+// Clears the error word of a handle's device-side waits (once, at the start of a public entry
+// point; the waits of every later launch OR into it).
+inline int lu_clear_err(lsx_lu* lu) {
+    return hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream) == hipSuccess ? LSX_OK : LSX_EHIP;
+}
 // Buffers for order lu->cap (0: lu->n).  With own_F false F is not allocated; with lookahead
 // false getrf runs on lu->stream alone and no CU-partitioned streams are made.
 int lu64_alloc(lsx_lu* lu);

@@ -24,7 +24,7 @@ static void launch_panel_t(lsx_lu* lu, T* F, int64_t ld, int64_t j0, int W) {
     a.dinfo = lu->dinfo;
     a.slots = lu->slots;
     a.seq = lu->seq;
-    a.timeout = lu->ctx->wall_khz * 5000;  // 5 s
+    a.timeout = wait_ticks(lu->ctx, 5.0);
     lu->seq += (uint32_t)W + 1u;
     const int pi = lu->prof.begin(lu->cur, PK_PANEL, (double)m_rows * W * W,
                                   2.0 * sizeof(T) * (double)m_rows * W);
@@ -166,7 +166,9 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
 }
 
 // X = A \ B with the resident factors (device buffers; FP64 right-hand sides, arithmetic in
-// FP64 whatever the factor type).  The lower solve writes lu->wy; X may alias B.
+// FP64 whatever the factor type).  The lower solve writes lu->wy; X may alias B.  A wait that
+// gives up sets lu->dinfo[1], which the public entry point clears once (lu_clear_err) and reads
+// after its last launch, so a timeout in any of its solves is reported.
 template <typename T>
 int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, int64_t nrhs, const double* dB,
                   int64_t ldb, double* dX, int64_t ldx) {
@@ -177,8 +179,7 @@ int getrs_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU, 
         LSX_HIP(ctx, hipMemsetAsync(lu->tmsg, 0, sizeof(uint64_t) * nblk * TMW, lu->stream));
         lu->tseq = 1;
     }
-    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
-    const int64_t timeout = ctx->wall_khz * 5000;
+    const int64_t timeout = wait_ticks(ctx, 5.0);
     for (int64_t c0 = 0; c0 < nrhs; c0 += TRHS) {
         const int nr = (int)std::min<int64_t>(TRHS, nrhs - c0);
         const uint32_t s1 = lu->tseq++, s2 = lu->tseq++;
@@ -214,8 +215,7 @@ int getrs_t_blocked(lsx_lu* lu, const T* F, int64_t ld, const T* DL, const T* DU
         LSX_HIP(ctx, hipMemsetAsync(lu->tmsg, 0, sizeof(uint64_t) * nblk * TMW, lu->stream));
         lu->tseq = 1;
     }
-    LSX_HIP(ctx, hipMemsetAsync(lu->dinfo + 1, 0, sizeof(int64_t), lu->stream));
-    const int64_t timeout = ctx->wall_khz * 5000;
+    const int64_t timeout = wait_ticks(ctx, 5.0);
     for (int64_t s = 0; s < nrhs; ++s) {
         const uint32_t s1 = lu->tseq++, s2 = lu->tseq++;
         const int pi = lu->prof.begin(lu->stream, PK_TRSV, 2.0 * n * n, sizeof(T) * (double)n * n);

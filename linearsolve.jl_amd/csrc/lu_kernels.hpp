@@ -143,8 +143,12 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
     __shared__ double s_candl[2][NW];
     __shared__ __align__(16) uint32_t s_rec[2][PMAX * RW];
 
-    // interchange of column k with pivot row grow (relabelling), pivots and info
+    // interchange of column k with pivot row grow (relabelling), pivots and info.  A live row
+    // always wins the pivot search, so grow lies in [k, m); only a record left stale by a wait
+    // that gave up (the factorization then reports LSX_ETIMEOUT) could hold another label, and it
+    // is replaced by k so that every later row index stays inside the matrix.
     auto pivot_bookkeeping = [&](int k, int grow, T pv) {
+        if (grow < k || (int64_t)grow >= a.m) grow = k;
         if (lr == grow) {
             live = false;
             lr = k;
@@ -302,10 +306,13 @@ __global__ __launch_bounds__(BS) void k_panel(PanelArgs<T> a) {
                         }
                     }
                     if (stale) {
+                        // give up at the bound, or at once when another wait of this
+                        // factorization already has (its results are invalid anyway)
                         if (wall_clock64() - t0 > a.timeout) {
                             atomicOr((unsigned long long*)&a.dinfo[1], 1ull);
                             break;
                         }
+                        if (ld_relaxed_u64((const uint64_t*)&a.dinfo[1]) != 0) break;
                         __builtin_amdgcn_s_sleep(2);
                     }
                 }
@@ -673,6 +680,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
                     atomicOr((unsigned long long*)err, 2ull);
                     break;
                 }
+                if (stale && ld_relaxed_u64((const uint64_t*)err) != 0) break;  // an earlier wait gave up
             }
         }
         // the two halves of a value sit in words w, w + 1 of threads tid, tid ^ 1
@@ -773,6 +781,7 @@ __global__ __launch_bounds__(256) void k_trsv_t(const T* __restrict F, int64_t l
                     atomicOr((unsigned long long*)err, 2ull);
                     break;
                 }
+                if (ld_relaxed_u64((const uint64_t*)err) != 0) break;  // an earlier wait gave up
             }
         }
         const uint32_t mine = (uint32_t)x;

@@ -14,6 +14,7 @@
 
 #include "gemm.hpp"
 #include "lu_driver.hpp"
+#include "residual.hpp"
 
 namespace lsx {
 
@@ -166,86 +167,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16(const float* __restrict Ag
     }
 }
 
-// mode 0: R = B - A X (FP64, nrhs <= TRHS columns) and, per column, partial max |r| and max |x|
-//         of this workgroup's rows: part[blk * 2 TRHS + {s, TRHS + s}];
-// mode 1: partial max_i sum_k |A[i,k]| (the inf-norm of A) into part[blk];
-// mode 2: R = A X.
-constexpr int RRB = 64;
-__global__ __launch_bounds__(256) void k_residual(const double* __restrict A, int64_t ld, int64_t n,
-                                                  const double* __restrict B, int64_t ldb, const double* __restrict X,
-                                                  int64_t ldx, double* __restrict R, int64_t ldr, int nrhs,
-                                                  double* __restrict part, int mode) {
-    const int rowsum = mode == 1;
-    const int64_t r0 = (int64_t)blockIdx.x * RRB;
-    const int tid = threadIdx.x, rr = tid & 63, g = tid >> 6;
-    const int64_t row = r0 + rr;
-    __shared__ double Xs[256][TRHS + 1];
-    __shared__ double Ps[4][RRB][TRHS + 1];
-    double acc[TRHS];
-#pragma unroll
-    for (int s = 0; s < TRHS; ++s) acc[s] = 0.0;
-    for (int64_t k0 = 0; k0 < n; k0 += 256) {
-        const int kc = (int)std::min<int64_t>(256, n - k0);
-        if (!rowsum) {
-            for (int idx = tid; idx < 256 * TRHS; idx += 256) {
-                const int kk = idx & 255, s = idx >> 8;
-                Xs[kk][s] = (kk < kc && s < nrhs) ? X[(int64_t)s * ldx + k0 + kk] : 0.0;
-            }
-        }
-        __syncthreads();
-        if (row < n) {
-            for (int kk = g; kk < kc; kk += 4) {
-                const double a = A[(k0 + kk) * ld + row];
-                if (rowsum) {
-                    acc[0] += fabs(a);
-                } else {
-#pragma unroll
-                    for (int s = 0; s < TRHS; ++s) acc[s] = fma(a, Xs[kk][s], acc[s]);
-                }
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int s = 0; s < TRHS; ++s) Ps[g][rr][s] = acc[s];
-    __syncthreads();
-    if (rowsum) {
-        if (tid == 0) {
-            double m = 0.0;
-            for (int i = 0; i < RRB; ++i)
-                if (r0 + i < n) m = fmax(m, Ps[0][i][0] + Ps[1][i][0] + Ps[2][i][0] + Ps[3][i][0]);
-            part[blockIdx.x] = m;
-        }
-        return;
-    }
-    __shared__ double mr[RRB][TRHS + 1];
-    __shared__ double mx[RRB][TRHS + 1];
-    for (int idx = tid; idx < RRB * TRHS; idx += 256) {
-        const int i = idx & (RRB - 1), s = idx >> 6;
-        const int64_t gr = r0 + i;
-        double rv = 0.0, xv = 0.0;
-        if (gr < n && s < nrhs) {
-            const double ax = Ps[0][i][s] + Ps[1][i][s] + Ps[2][i][s] + Ps[3][i][s];
-            rv = (mode == 0) ? B[(int64_t)s * ldb + gr] - ax : ax;
-            R[(int64_t)s * ldr + gr] = rv;
-            xv = X[(int64_t)s * ldx + gr];
-        }
-        mr[i][s] = fabs(rv);
-        mx[i][s] = fabs(xv);
-    }
-    if (mode == 2) return;
-    __syncthreads();
-    if (tid < TRHS) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < RRB; ++i) {
-            a = fmax(a, mr[i][tid]);
-            b = fmax(b, mx[i][tid]);
-        }
-        part[(int64_t)blockIdx.x * 2 * TRHS + tid] = a;
-        part[(int64_t)blockIdx.x * 2 * TRHS + TRHS + tid] = b;
-    }
-}
-
 // Y += alpha X, column-major n x nrhs.
 __global__ void k_axpy_cols(double* __restrict Y, int64_t ldy, const double* __restrict X, int64_t ldx, int64_t n,
                             int nrhs, double alpha) {
@@ -301,10 +222,9 @@ int mixed_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&m->DU, sizeof(float) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&m->anorm, sizeof(double)));
     LSX_HIP(ctx, hipMalloc(&m->scale, sizeof(float) * 2 * (lu->n + 64)));
-    const int64_t nb = cdiv(n, RRB);
-    LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64)));
-    LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * std::max<int64_t>(nb * 2 * TRHS, cdiv(n, 256) * 64),
-                               hipHostMallocDefault));
+    const int64_t pw = std::max<int64_t>(resid_part_words(n), cdiv(n, 256) * 64);
+    LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * pw));
+    LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * pw, hipHostMallocDefault));
     return LSX_OK;
 }
 
@@ -356,9 +276,8 @@ int mixed_getrf(lsx_lu* lu) {
     MixedState* m = ms(lu);
     const int64_t n = lu->n;
     // ||A||_inf in FP64 (for the backward error), then the working copy in FP32
-    const int64_t nb = cdiv(n, RRB);
-    hipLaunchKernelGGL(k_residual, dim3((unsigned)nb), dim3(256), 0, lu->stream, lu->F, lu->ld, n, (const double*)nullptr,
-                       (int64_t)0, (const double*)nullptr, (int64_t)0, (double*)nullptr, (int64_t)0, 1, m->part, 1);
+    const int64_t nb = resid_blocks(n);
+    resid_launch(lu->stream, lu->F, lu->ld, n, nullptr, 0, 1, nullptr, 0, nullptr, 0, m->part, RES_ROWSUM);
     LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb, hipMemcpyDeviceToHost, lu->stream));
     const int pc = lu->prof.begin(lu->stream, PK_CONVERT, 0.0, 12.0 * (double)n * n);
     hipLaunchKernelGGL(k_cast_f32, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 64), (unsigned)n), dim3(256), 0,
@@ -375,7 +294,7 @@ int mixed_getrf(lsx_lu* lu) {
         rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gt, gt);
     }
     double an = 0.0;
-    for (int64_t i = 0; i < nb; ++i) an = std::max(an, m->hpart[i]);
+    for (int64_t i = 0; i < nb; ++i) an = nanmax(an, m->hpart[i]);
     m->anorm_h = an;
     return rc;
 }
@@ -413,26 +332,28 @@ static int ensure_mixed_work(lsx_lu* lu, int64_t nrhs) {
     return lu_ensure_work(lu, cap);
 }
 
-// r = b - A x for columns [c0, c0 + nr) and the per-column backward errors.
-static int residual(lsx_lu* lu, int64_t c0, int nr, const double* bnorm, double* be) {
+// r = b - A x for columns [c0, c0 + nr) and the per-column backward errors (NaN stays NaN, so
+// it never passes a tolerance test); with bnorm null, bmax receives max |b| per column.
+static int residual(lsx_lu* lu, int64_t c0, int nr, const double* bnorm, double* be, double* bmax = nullptr) {
     lsx_ctx* ctx = lu->ctx;
     MixedState* m = ms(lu);
-    const int64_t n = lu->n, nb = cdiv(n, RRB);
+    const int64_t n = lu->n;
     const int pi = lu->prof.begin(lu->stream, PK_REFINE, 2.0 * n * n * nr, 8.0 * (double)n * n);
-    hipLaunchKernelGGL(k_residual, dim3((unsigned)nb), dim3(256), 0, lu->stream, lu->F, lu->ld, n,
-                       (const double*)(m->bw + c0 * n), n, (const double*)(m->xw + c0 * n), n, m->r + c0 * n, n, nr,
-                       m->part, 0);
+    resid_launch(lu->stream, lu->F, lu->ld, n, m->xw + c0 * n, n, nr, m->bw + c0 * n, n, m->r + c0 * n, n, m->part,
+                 RES_RESIDUAL);
     lu->prof.end(lu->stream, pi);
-    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb * 2 * TRHS, hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipGetLastError());
+    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * resid_part_words(n), hipMemcpyDeviceToHost,
+                                lu->stream));
     LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    double stv[RNR][RSTAT];
+    resid_combine(m->hpart, n, nr, stv);
     for (int s = 0; s < nr; ++s) {
-        double rn = 0.0, xn = 0.0;
-        for (int64_t b = 0; b < nb; ++b) {
-            rn = std::max(rn, m->hpart[b * 2 * TRHS + s]);
-            xn = std::max(xn, m->hpart[b * 2 * TRHS + TRHS + s]);
-        }
+        if (bmax) bmax[s] = stv[s][3];
+        if (!bnorm) continue;
+        const double rn = stv[s][0], xn = stv[s][2];
         const double den = m->anorm_h * xn + bnorm[c0 + s];
-        be[s] = den > 0.0 ? rn / den : (rn > 0.0 ? INFINITY : 0.0);
+        be[s] = den > 0.0 ? rn / den : (rn > 0.0 || rn != rn ? rn / den : 0.0);
     }
     return LSX_OK;
 }
@@ -487,12 +408,9 @@ static int gmres_ir_correction(lsx_lu* lu, int64_t s, int maxit, double rtol) {
     std::vector<double> H((size_t)(M + 1) * M, 0.0), cs(M), sn(M), z(M + 1, 0.0), h(M + 1), h2(M + 1);
     z[0] = beta;
     int k = 0;
-    const int nbr = (int)cdiv(n, RRB);
     for (; k < M; ++k) {
         // w = M^-1 A v_k
-        hipLaunchKernelGGL(k_residual, dim3((unsigned)nbr), dim3(256), 0, lu->stream, lu->F, lu->ld, n,
-                           (const double*)nullptr, (int64_t)0, (const double*)(m->V + k * n), n, m->w, n, 1,
-                           m->part, 2);
+        resid_launch(lu->stream, lu->F, lu->ld, n, m->V + k * n, n, 1, nullptr, 0, m->w, n, m->part, RES_PRODUCT);
         rc = mixed_getrs_dev(lu, 1, m->w, n, m->w, n);
         if (rc) return rc;
         // CGS2
@@ -563,6 +481,8 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
     const int64_t n = lu->n;
     int rc = ensure_mixed_work(lu, nrhs);
     if (rc) return rc;
+    rc = lu_clear_err(lu);
+    if (rc) return fail(ctx, rc, "lsx_lu_refine: cannot clear the wait error word");
     LSX_HIP(ctx, hipMemcpy2DAsync(m->bw, sizeof(double) * n, B, sizeof(double) * ldb, sizeof(double) * n, nrhs,
                                   hipMemcpyDefault, lu->stream));
     // ||b_j||_inf (host), from the residual kernel with x = 0
@@ -573,15 +493,8 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
     bool all_conv = true;
     for (int64_t c0 = 0; c0 < nrhs; c0 += TRHS) {
         const int nr = (int)std::min<int64_t>(TRHS, nrhs - c0);
-        std::vector<double> tmp(nr);
-        rc = residual(lu, c0, nr, bnorm.data(), tmp.data());  // r = b, be = ||b|| / ||b||
+        rc = residual(lu, c0, nr, nullptr, nullptr, bnorm.data() + c0);  // x = 0: max |b| per column
         if (rc) return rc;
-        for (int s = 0; s < nr; ++s) {
-            double rn = 0.0;
-            const int64_t nb = cdiv(n, RRB);
-            for (int64_t b = 0; b < nb; ++b) rn = std::max(rn, m->hpart[b * 2 * TRHS + s]);
-            bnorm[c0 + s] = rn;
-        }
         // x0 = LU \ b
         rc = mixed_getrs_dev(lu, nr, m->bw + c0 * n, n, m->xw + c0 * n, n);
         if (rc) return rc;
@@ -593,15 +506,15 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
             rc = residual(lu, c0, nr, bnorm.data(), be.data());
             if (rc) return rc;
             double worst_c = 0.0;
-            for (int s = 0; s < nr; ++s) worst_c = std::max(worst_c, be[s]);
+            for (int s = 0; s < nr; ++s) worst_c = nanmax(worst_c, be[s]);
             ++outer;
             if (worst_c <= tol) {
                 conv = true;
-                worst = std::max(worst, worst_c);
+                worst = nanmax(worst, worst_c);
                 break;
             }
             if (it == max_iters) {
-                worst = std::max(worst, worst_c);
+                worst = nanmax(worst, worst_c);
                 break;
             }
             if (mode == 2) {
@@ -612,12 +525,23 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
                     mode_used = 1;
                     stall = 0;
                 } else if (stall >= 3 && use_gmres && lu->dtype == LSX_BF16 && !escalated) {
-                    // GMRES-IR stalls too (u_bf16 * cond(A) >= 1): refactor in FP32, keep x
+                    // GMRES-IR stalls too (u_bf16 * cond(A) >= 1): refactor in FP32, keep x.
+                    // The refactorization clears the wait error word, so a wait that gave up in
+                    // the solves so far is reported first.
+                    LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t),
+                                                hipMemcpyDeviceToHost, lu->stream));
+                    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+                    if (lu->hinfo[1]) return fail(ctx, LSX_ETIMEOUT, "lsx_lu_refine: a block wait exceeded its bound");
                     const int saved = lu->dtype;
                     lu->dtype = LSX_F32;
                     rc = mixed_getrf(lu);
                     lu->dtype = saved;
                     if (rc < 0) return rc;
+                    if (rc > 0) {
+                        // the FP32 factors are singular: no refinement is possible (LAPACK info)
+                        lu->info = rc;
+                        return rc;
+                    }
                     escalated = 1;
                     use_gmres = 0;
                     stall = 0;

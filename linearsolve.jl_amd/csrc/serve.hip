@@ -448,6 +448,7 @@ void run_dense(lsx_serve* s, DenseLane& L, Job* j) {
         lu->F = nullptr;
     }
     j->res.status = rc;
+    if (rc != LSX_OK) j->res.retcode = LSX_RET_FAILURE;  // no valid result
 }
 
 void dense_worker(lsx_serve* s, int lane) {
@@ -481,6 +482,7 @@ int take_gmres(lsx_serve* s, std::vector<Job*>& group) {
     group.push_back(f);
     int reserve = f->coop;
     int64_t total = f->coop;
+    bool multi = f->coop > 1;  // the group holds a solve of several workgroups
     int scanned = 0;
     for (auto it = q.begin(); it != q.end() && (int)group.size() < MAXG && scanned < 512; ++scanned) {
         Job* j = *it;
@@ -489,8 +491,16 @@ int take_gmres(lsx_serve* s, std::vector<Job*>& group) {
             continue;
         }
         const int grow = std::max(reserve, j->coop) - reserve;
-        // about two pool turns of work per launch: later arrivals join the next launch soon
-        if (reserve + grow > avail || total >= 2 * (int64_t)s->gmres_pool) break;
+        if (reserve + grow > avail) break;
+        // A solve of several workgroups must start with all of them: its first all-reduce waits
+        // for every rank, and a rank that only starts when an earlier solve ends could wait past
+        // the bound.  So a group that holds one never needs more workgroups than its launch's pool
+        // (every work number is taken at once); groups of one-workgroup solves, which wait on
+        // nobody, take about two pool turns of work, so later arrivals join the next launch soon.
+        const bool m2 = multi || j->coop > 1;
+        const int64_t pool = std::max<int64_t>(reserve + grow, s->gmres_pool);
+        if (m2 ? total + j->coop > pool : total >= 2 * (int64_t)s->gmres_pool) break;
+        multi = m2;
         off = s->heap.alloc(j->footprint);
         if (off == SIZE_MAX) break;
         j->off = off;
@@ -575,7 +585,7 @@ void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
         a.red = g.red;
         a.bar = reinterpret_cast<uint32_t*>(g.red + 4 * (int64_t)j->plan.P);
         a.out = L.d_out + m;
-        a.timeout = ctx->wall_khz * 20000;  // 20 s
+        a.timeout = wait_ticks(ctx, 20.0);
         L.h_first[m] = total;
         total += j->plan.P;
         widest = std::max(widest, j->plan.P);
@@ -600,13 +610,16 @@ void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
         }
     }
     if (hipStreamSynchronize(st) != hipSuccess && rc == LSX_OK) rc = LSX_EHIP;
-    for (int i = 0; i < cnt; ++i) group[i]->res.status = rc != LSX_OK ? rc : (bad[i] ? LSX_EINVAL : LSX_OK);
+    for (int i = 0; i < cnt; ++i) {
+        group[i]->res.status = rc != LSX_OK ? rc : (bad[i] ? LSX_EINVAL : LSX_OK);
+        group[i]->res.retcode = LSX_RET_FAILURE;  // until a valid result is recorded below
+    }
     for (int m = 0; m < nl && rc == LSX_OK; ++m) {
         Job* j = group[idx[m]];
-        if (L.h_out[m].err) j->res.status = LSX_ETIMEOUT;
-        j->res.retcode = L.h_out[m].retcode;
         j->res.iters = L.h_out[m].iters;
         j->res.resid = L.h_out[m].resid;
+        if (L.h_out[m].err) j->res.status = LSX_ETIMEOUT;  // a wait gave up: the result is invalid
+        else j->res.retcode = L.h_out[m].retcode;
     }
 }
 
