@@ -43,8 +43,16 @@ def test_gmres_infeasible_matches_oracle(lsx, case, restart):
     assert ref["retcode"] == O.RET_INFEASIBLE
     assert sol.retcode == lsx.ReturnCode.Infeasible
     assert sol.iters == ref["iters"]
-    np.testing.assert_allclose(sol.resid, ref["resid"], rtol=1e-12, atol=1e-15)
+    # y_k is zeroed where |R_kk| <= btol, so the iterate does not depend on rounding
     np.testing.assert_allclose(np.asarray(sol.u), ref["u"], rtol=1e-12, atol=1e-15)
+    # The returned resid is |zeta_{k+1}| = |s_k zeta_k|, and at this breakdown the last Givens
+    # rotation is taken between R_kk and Hbis, which are both rounding noise (about 1e-16 in
+    # cases 1 and 2), so s_k is any value in [0, 1] that the summation order leaves.  What is fixed
+    # is the bound |zeta_{k+1}| <= |zeta_k|: the estimate after iters - 1 steps.
+    prev = O.gmres(A.rowptr, A.colind, A.vals, b, memory=20, restart=restart, maxiters=ref["iters"] - 1)
+    assert 0.0 <= sol.resid <= prev["resid"] * (1 + 1e-12)
+    if case == 0:  # exact zeros throughout: the rotation is the identity and the estimate is exact
+        assert sol.resid == ref["resid"]
     # KrylovJL_GMRES maps an inconsistent system to Failure (src/iterative_wrappers.jl:733-745)
     solk = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XKrylovJL_GMRES(gmres_restart=20), maxiters=10)
     assert solk.retcode == lsx.ReturnCode.Failure
