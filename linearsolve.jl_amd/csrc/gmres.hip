@@ -12,8 +12,8 @@
 //   back-substitution R y = z with zero for |R_ii| <= btol (:576-585), x_r += sum y_i v_i,
 //     x_r = Pr \ x_r, x += x_r (:587-595); retcodes (:603-606).
 //
-// Layout: workgroup w of P owns rows [w R, (w+1) R), R = 1024 * RPT; thread t holds rows
-// w R + t + i 1024 (i < RPT) of q and of the current basis vector in registers.  Every
+// Layout: workgroup w of P owns rows [w R, (w+1) R), R = GBS * RPT (GBS = 512 threads); thread
+// t holds rows w R + t + i GBS (i < RPT) of q and of the current basis vector in LDS.  Every
 // workgroup keeps its own copy of the Hessenberg/Givens scalars (R, c, s, z) in LDS and
 // computes them identically from all-reduced values, so all workgroups take the same branches.
 // All-reduces combine per-workgroup partial sums in a fixed order (deterministic, the same

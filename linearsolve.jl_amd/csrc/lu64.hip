@@ -4,8 +4,9 @@
 // and of the reference's own restatement src/blocked_lufact.jl:658-679:
 //   outer block NBO = 128 columns (the K of the trailing MFMA update, which holds ~2/3 n^3 flops);
 //   inside it, inner panels of PW = 32 columns are factored by a register-resident kernel whose
-//   workgroups each hold 1024 rows x 32 columns, exchanging one pivot candidate record per
-//   column through flag-in-word messages (no fences, one memory round trip per column).
+//   workgroups each hold 512 rows x 32 columns (k_panel<T, 512>; one workgroup of 256 rows when
+//   the panel has at most 256), exchanging one pivot candidate record per column through
+//   flag-in-word messages (no fences, one memory round trip per column).
 // Pivot search: maximum |a| with NaN ignored, first (smallest) row on ties
 // (src/blocked_lufact.jl:38-54); zero pivot -> info = first such column, factorization
 // continues (src/generic_lufact.jl:71-141).  Scaling multiplies by the reciprocal of the pivot
