@@ -299,16 +299,37 @@ int mixed_getrf(lsx_lu* lu) {
     return rc;
 }
 
+// dst (n x ncols, ldd, FP64) = src (FP32, lds)
+__global__ void k_widen(const float* __restrict src, int64_t lds, double* __restrict dst, int64_t ldd, int64_t n) {
+    const int64_t col = blockIdx.y;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[col * ldd + i] = (double)src[col * lds + i];
+}
+
 int mixed_get_factors(lsx_lu* lu, double* F, int64_t ldf) {
-    // widen the FP32 factors into the caller's FP64 buffer, one column at a time through lu->wy
+    // widen the FP32 factors on the device: straight into F when it is memory of this GPU, else
+    // through lu->wb in blocks of columns, one copy per block
     lsx_ctx* ctx = lu->ctx;
-    std::vector<float> col(lu->n);
-    std::vector<double> colw(lu->n);
-    for (int64_t j = 0; j < lu->n; ++j) {
-        LSX_HIP(ctx, hipMemcpy(col.data(), lu->F32 + j * lu->ld32, sizeof(float) * lu->n, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < lu->n; ++i) colw[i] = col[i];
-        LSX_HIP(ctx, hipMemcpy(F + j * ldf, colw.data(), sizeof(double) * lu->n, hipMemcpyDefault));
+    const int64_t n = lu->n;
+    const hipStream_t st = lu->stream;
+    const unsigned gx = (unsigned)std::min<int64_t>(cdiv(n, 256), 64);
+    if (is_device_ptr(ctx->device, F)) {
+        hipLaunchKernelGGL(k_widen, dim3(gx, (unsigned)n), dim3(256), 0, st, (const float*)lu->F32, lu->ld32, F, ldf, n);
+        LSX_HIP(ctx, hipGetLastError());
+    } else {
+        const int64_t cols = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(64 << 20) / (8 * n)));
+        int rc = lu_ensure_work(lu, cols);
+        if (rc) return rc;
+        for (int64_t c0 = 0; c0 < n; c0 += cols) {
+            const int64_t nc = std::min<int64_t>(cols, n - c0);
+            hipLaunchKernelGGL(k_widen, dim3(gx, (unsigned)nc), dim3(256), 0, st, (const float*)(lu->F32 + c0 * lu->ld32),
+                               lu->ld32, lu->wb, n, n);
+            LSX_HIP(ctx, hipGetLastError());
+            LSX_HIP(ctx, hipMemcpy2DAsync(F + c0 * ldf, sizeof(double) * ldf, lu->wb, sizeof(double) * n,
+                                          sizeof(double) * n, nc, hipMemcpyDeviceToHost, st));
+        }
     }
+    LSX_HIP(ctx, hipStreamSynchronize(st));
     return LSX_OK;
 }
 
