@@ -110,6 +110,19 @@ int lsx_lu_solve_adjoint(lsx_lu* lu, int64_t nrhs, const double* B, int64_t ldb,
 int lsx_lu_residual_norm(lsx_lu* lu, const double* A, int64_t lda, int64_t nrhs, const double* B, int64_t ldb,
                          const double* X, int64_t ldx, double* norms);
 
+/* The same two norms as lsx_lu_residual_norm (bit for bit: the same kernel and summation order)
+ * with no handle and no device allocation, for callers that cannot allocate: code running
+ * between the kernels of a captured or tight serving loop, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  A, B and X must be device memory of ctx's GPU (a host array
+ * would need a staging copy) and n <= LSX_NOALLOC_MAX_N; otherwise LSX_EINVAL.  norms is host
+ * memory.  The work runs on the GPU's null stream (ordered after the caller's work on blocking
+ * streams) and the call returns when it is done.  Its partial sums live in a scratch array of
+ * 656 KB that each GPU holds from lsx_ctx_create until the process ends; calls on one GPU share
+ * it, so calls that overlap on one GPU (from several host threads) race. */
+#define LSX_NOALLOC_MAX_N 65536
+int lsx_residual_norm_noalloc(lsx_ctx* ctx, int64_t n, const double* A, int64_t lda, int64_t nrhs,
+                              const double* B, int64_t ldb, const double* X, int64_t ldx, double* norms);
+
 /* Per-kernel-class device timing of a handle (HIP events around each launch while enabled).
  * Classes: 0 panel, 1 row interchanges + TRSM, 2 trailing MFMA update (128 x 128 tiles),
  * 3 permutation lists, 4 triangular solves, 5 refinement residual/update, 6 low-precision

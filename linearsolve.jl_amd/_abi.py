@@ -87,6 +87,7 @@ SIGNATURES = [
     ("lsx_lu_solve", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64]),
     ("lsx_lu_solve_adjoint", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64]),
     ("lsx_lu_residual_norm", ctypes.c_int, [_P, _P, _I64, _I64, _P, _I64, _P, _I64, _P]),
+    ("lsx_residual_norm_noalloc", ctypes.c_int, [_P, _I64, _P, _I64, _I64, _P, _I64, _P, _I64, _P]),
     ("lsx_lu_profile", ctypes.c_int, [_P, _I32, ctypes.POINTER(Profile)]),
     ("lsx_lu_refine", ctypes.c_int, [_P, _I64, _P, _I64, _P, _I64, _D, _I32, _I32, ctypes.POINTER(RefineStats)]),
     ("lsx_batch_create", ctypes.c_int, [_P, _I64, _P, ctypes.c_int, ctypes.POINTER(_P)]),

@@ -8,6 +8,14 @@
 #include "residual.hpp"
 
 namespace lsx {
+// Scratch memory of lsx_residual_norm_noalloc: the per-block statistics of one residual launch
+// for n <= LSX_NOALLOC_MAX_N, then the two running sums of squares.  A file-scope device array,
+// so that entry point never allocates; every GPU the library runs on holds its own copy
+// (656 KB) until the process ends.  Calls on one GPU share it: overlapping calls race.
+constexpr int64_t NOALLOC_PART_WORDS = resid_part_words(LSX_NOALLOC_MAX_N);
+constexpr int64_t NOALLOC_WORDS = NOALLOC_PART_WORDS + 32;  // + the sums (own 256-byte line)
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+
 int fail(lsx_ctx* ctx, int code, const char* fmt, ...) {
     if (!ctx) return code;
     char buf[1024];
@@ -143,6 +151,14 @@ int lsx_ctx_create(int device, lsx_ctx** out) {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         c->wall_khz = khz;
+    // the device address of the no-allocation scratch array on this GPU (each GPU holds its own
+    // copy); host code uses this address, never the array's name
+    const hipError_t e = hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !c->scratch) {
+        (void)hipGetLastError();
+        delete c;
+        return LSX_EHIP;
+    }
     *out = c;
     return LSX_OK;
 }
@@ -362,6 +378,44 @@ int lsx_lu_residual_norm(lsx_lu* lu, const double* A, int64_t lda, int64_t nrhs,
     }
     norms[0] = std::sqrt(ssr);
     norms[1] = std::sqrt(ssb);
+    return LSX_OK;
+}
+
+int lsx_residual_norm_noalloc(lsx_ctx* ctx, int64_t n, const double* A, int64_t lda, int64_t nrhs,
+                              const double* B, int64_t ldb, const double* X, int64_t ldx, double* norms) {
+    if (!ctx || !A || !B || !X || !norms || n < 1 || nrhs < 0) return LSX_EINVAL;
+    if (n > LSX_NOALLOC_MAX_N)
+        return fail(ctx, LSX_EINVAL, "lsx_residual_norm_noalloc: n = %lld exceeds the scratch bound %d",
+                    (long long)n, LSX_NOALLOC_MAX_N);
+    if (lda < n || ldb < n || ldx < n)
+        return fail(ctx, LSX_EINVAL, "lsx_residual_norm_noalloc: lda/ldb/ldx < n");
+    if (!ctx->scratch) return fail(ctx, LSX_ESTATE, "lsx_residual_norm_noalloc: no scratch address");
+    DeviceGuard g(ctx);
+    // nothing is staged (that would allocate): A, B and X must be memory of this GPU
+    if (!is_device_ptr(ctx->device, A) || !is_device_ptr(ctx->device, B) || !is_device_ptr(ctx->device, X))
+        return fail(ctx, LSX_EINVAL, "lsx_residual_norm_noalloc: A, B and X must be device memory of GPU %d",
+                    ctx->device);
+    if (nrhs == 0) {
+        norms[0] = norms[1] = 0.0;
+        return LSX_OK;
+    }
+    double* part = ctx->scratch;
+    double* acc = ctx->scratch + NOALLOC_PART_WORDS;
+    const hipStream_t st = nullptr;  // the null stream: ordered after the caller's blocking work
+    for (int64_t c0 = 0; c0 < nrhs; c0 += RNR) {
+        const int nr = (int)std::min<int64_t>(RNR, nrhs - c0);
+        // statistics only (no R): every block writes all of its statistics words, so nothing in
+        // the scratch needs clearing, and the first sum launch starts the totals at zero
+        resid_launch(st, A, lda, n, X + c0 * ldx, ldx, nr, B + c0 * ldb, ldb, nullptr, 0, part, RES_RESIDUAL);
+        hipLaunchKernelGGL(k_resid_sumsq, dim3(1), dim3(64), 0, st, (const double*)part, resid_blocks(n), nr, acc,
+                           c0 == 0 ? 1 : 0);
+        LSX_HIP(ctx, hipGetLastError());
+    }
+    double h[2];
+    LSX_HIP(ctx, hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, st));
+    LSX_HIP(ctx, hipStreamSynchronize(st));
+    norms[0] = std::sqrt(h[0]);
+    norms[1] = std::sqrt(h[1]);
     return LSX_OK;
 }
 

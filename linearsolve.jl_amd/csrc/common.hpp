@@ -16,6 +16,7 @@ struct lsx_ctx {
     int num_cus = 0;
     int64_t wall_khz = 100000;  // device wall clock rate (kHz), for bounded device-side waits
     double wait_bound_s = 0.0;  // > 0: bound of every later device-side wait (seconds); 0: defaults
+    double* scratch = nullptr;  // this GPU's address of the no-allocation scratch array (api.hip)
     std::mutex mu;
     std::string err;
 };

@@ -23,7 +23,8 @@ enum { RES_RESIDUAL = 0, RES_ROWSUM = 1, RES_PRODUCT = 2 };
 // max that propagates NaN (a NaN residual must not read as a small one)
 __host__ __device__ inline double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
-// mode RES_RESIDUAL: R = B - A X, statistics into part[blk][RSTAT][RNR];
+// mode RES_RESIDUAL: R = B - A X (R may be null: statistics only), statistics into
+//                    part[blk][RSTAT][RNR];
 // mode RES_PRODUCT:  R = A X;
 // mode RES_ROWSUM:   part[blk] = max over the block's rows of sum_k |A[i, k]| (X, B, R unused).
 template <int NR>
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(1024) void k_resid(const double* __restrict A, int6
             r = b - ax;
             x = X[(int64_t)s * ldx + row];
         }
-        R[(int64_t)s * ldr + row] = r;
+        if (R) R[(int64_t)s * ldr + row] = r;
     }
     if (mode == RES_PRODUCT) return;
     // statistics of the block per right-hand side: wave s holds column s, one row per lane
@@ -123,9 +124,9 @@ __global__ __launch_bounds__(1024) void k_resid(const double* __restrict A, int6
     }
 }
 
-inline int64_t resid_blocks(int64_t n) { return (n + RROWS - 1) / RROWS; }
+constexpr int64_t resid_blocks(int64_t n) { return (n + RROWS - 1) / RROWS; }
 // doubles of the statistics area of one launch
-inline int64_t resid_part_words(int64_t n) { return resid_blocks(n) * RSTAT * RNR; }
+constexpr int64_t resid_part_words(int64_t n) { return resid_blocks(n) * RSTAT * RNR; }
 
 // One launch over nrhs <= RNR right-hand sides (mode as k_resid).
 inline void resid_launch(hipStream_t st, const double* A, int64_t lda, int64_t n, const double* X, int64_t ldx,
@@ -152,6 +153,35 @@ inline void resid_combine(const double* hpart, int64_t n, int nrhs, double (*out
             v[4] += p[4 * RNR + s];
         }
         for (int k = 0; k < RSTAT; ++k) out[s][k] = v[k];
+    }
+}
+
+// The two sums of squares of a launch's statistics (||R||^2 and ||B||^2 over its nrhs columns),
+// added to acc[0], acc[1] (first != 0: acc starts at zero), in the order resid_combine and the
+// host loops over columns use, so the device and host totals agree bit for bit.  One workgroup.
+static __global__ __launch_bounds__(64) void k_resid_sumsq(const double* __restrict part, int64_t nb, int nrhs,
+                                                    double* __restrict acc, int first) {
+    __shared__ double t[2][RNR];
+    const int s = threadIdx.x;
+    if (s < nrhs && s < RNR) {
+        double sr = 0.0, sb = 0.0;
+        for (int64_t b = 0; b < nb; ++b) {
+            const double* p = part + b * RSTAT * RNR;
+            sr += p[1 * RNR + s];
+            sb += p[4 * RNR + s];
+        }
+        t[0][s] = sr;
+        t[1][s] = sb;
+    }
+    __syncthreads();
+    if (s == 0) {
+        double a0 = first ? 0.0 : acc[0], a1 = first ? 0.0 : acc[1];
+        for (int k = 0; k < nrhs && k < RNR; ++k) {
+            a0 += t[0][k];
+            a1 += t[1][k];
+        }
+        acc[0] = a0;
+        acc[1] = a1;
     }
 }
 
