@@ -6,6 +6,9 @@
  *   - pivots are 1-based int64 LAPACK swap sequences (ipiv[k] = row interchanged with row k);
  *   - every pointer argument may point to host memory or to device memory of the context's GPU
  *     (the library detects which), except where a function says "device";
+ *   - device arrays are read and written on the library's own non-blocking streams, which do not
+ *     wait for the caller's streams: an input must be complete, and an output no longer in use,
+ *     when the call is made (synchronize the stream that produced it first);
  *   - every function returns int32: 0 = success, > 0 = LAPACK-style info (1-based index of the
  *     first exactly-zero pivot; the factorization still ran to completion), < 0 = error
  *     (one of lsx_status below); lsx_last_error() then returns a message.  No C++ exception

@@ -58,6 +58,17 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+def _ready(t):
+    """Waits for the work torch has queued on the tensor's device.  The library reads and writes
+    device arrays on its own non-blocking streams, which do not wait for torch's stream: an input
+    still being written there (a transposed copy just made, a fresh randn) would otherwise be read
+    half-written, and an output still being read there overwritten early."""
+    import torch
+    if t.is_cuda:
+        torch.cuda.current_stream(t.device).synchronize()
+    return t
+
+
 def _colmajor(x):
     """Returns (keepalive, pointer) of a column-major float64 copy/view of a vector or matrix."""
     if _is_torch(x):
@@ -67,7 +78,7 @@ def _colmajor(x):
             t = t.t().contiguous()  # row-major storage of A^T == column-major storage of A
         else:
             t = t.contiguous()
-        return t, t.data_ptr()
+        return _ready(t), t.data_ptr()
     a = np.asfortranarray(np.asarray(x, dtype=np.float64))
     return a, a.ctypes.data
 
@@ -83,7 +94,7 @@ def _empty_like_rhs(b):
 
 def _ptr_out(u):
     if _is_torch(u):
-        return u.data_ptr()
+        return _ready(u).data_ptr()
     return u.ctypes.data
 
 
@@ -312,8 +323,8 @@ class _LUHandle:
             if u.dim() == 2 and not (u.stride(0) == 1 and u.stride(1) == self.n):
                 import torch
                 t = torch.empty((u.shape[1], u.shape[0]), dtype=torch.float64, device=u.device).t()
-                return t, t.data_ptr(), self.n, True
-            return u, u.data_ptr(), self.n, False
+                return t, _ptr_out(t), self.n, True
+            return u, _ptr_out(u), self.n, False
         if u.ndim == 2 and not u.flags.f_contiguous:
             t = np.empty(u.shape, dtype=np.float64, order="F")
             return t, t.ctypes.data, self.n, True
@@ -349,7 +360,7 @@ class _LUHandle:
         keep, p = _colmajor(b)
         nr = _ncols(b)
         st = _abi.RefineStats()
-        up = u.data_ptr() if _is_torch(u) else _ptr_out(u)
+        up = _ptr_out(u)
         info = self.ctx.check(_abi.lib().lsx_lu_refine(self.h, nr, p, self.n, up, self.n, tol, max_iters, mode,
                                                        ctypes.byref(st)), "lsx_lu_refine")
         return info, st

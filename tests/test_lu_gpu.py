@@ -337,3 +337,32 @@ def test_refactorization_reuses_device_handle(lsx):
     cache.b = B
     cache.u = np.zeros((n, 4))
     np.testing.assert_allclose(lsx.solve_(cache).u, np.linalg.solve(A2, B), rtol=1e-12)
+
+
+@pytest.mark.parametrize("alg", ["lu", "mixed"])
+def test_torch_inputs_wait_for_torch_stream(lsx, alg):
+    """The library reads device arrays on its own non-blocking streams, which do not wait for
+    torch's.  The Python layer synchronizes torch's stream before each call, so an input that
+    torch is still producing (here A * 1.0 and its column-major copy, queued behind long products)
+    is read complete, not half-written."""
+    import torch
+    n = 2048
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    A = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    b = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        busy = torch.randn(4096, 4096, dtype=torch.float64, device="cuda", generator=g) * 1e-3
+        for _ in range(6):
+            busy = busy @ busy  # keeps torch's stream busy for tens of milliseconds
+        Ai = A * 1.0            # queued behind the products
+        bi = b * 1.0
+        a = lsx.MI355XLUFactorization() if alg == "lu" else lsx.MI355XMixedLUFactorization(precision="fp32")
+        sol = lsx.solve(lsx.LinearProblem(Ai, bi), a)
+        assert sol.retcode == lsx.ReturnCode.Success
+        x = sol.u
+        r = b - A @ x
+        be = float(r.abs().max() / (A.abs().sum(dim=1).max() * x.abs().max() + b.abs().max()))
+        assert be <= 1e-13, (rep, be)
+        del busy

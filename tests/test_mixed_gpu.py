@@ -86,3 +86,51 @@ def test_pivots_match_lapack_fp32(lsx):
     assert h.factor(A) == 0
     _, ipiv = h.factors()
     np.testing.assert_array_equal(ipiv, lapack.dgetrf(A)[1] + 1)
+
+
+@pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
+def test_conditioned_full_size(lsx, cond):
+    """The mixed-precision workload at its full size: n = 16384, A = Q1 diag(s) Q2^T with s
+    log-spaced from 1 to 1/cond, factors in FP32 and in BF16, refined to a normwise backward error
+    <= 1e-13; GMRES-IR at cond 1e6.  The forward error is checked against LAPACK's FP64 solve of
+    the same system (scipy's OpenBLAS dgetrf/dgetrs, the library LUFactorization calls): two
+    solutions with backward errors <= 1e-13 of a system whose smallest singular value is 1/cond
+    differ by at most about cond * 1e-13 relative."""
+    import scipy.linalg as sla
+    import torch
+    from linearsolve_jl_amd import problems as P
+    n = 16384
+    print(f"mixed full size n={n} cond={cond:.0e}: generating", flush=True)
+    A = P.conditioned(n, cond, device="cuda")  # row-major (n, n) tensor of A
+    g = torch.Generator(device="cuda")
+    g.manual_seed(31)
+    b = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    Ah = A.cpu().numpy()
+    bh = b.cpu().numpy()
+    lu, piv = sla.lu_factor(Ah, check_finite=False)
+    x64 = sla.lu_solve((lu, piv), bh, check_finite=False)
+    del lu
+    nA = float(torch.linalg.matrix_norm(A, ord=float("inf")))
+
+    def berr(x):
+        r = b - A @ x
+        return float(r.abs().max() / (nA * x.abs().max() + b.abs().max()))
+
+    runs = [("fp32", "gmres-ir" if cond >= 1e6 else "auto"), ("bf16", "auto")]
+    for precision, refinement in runs:
+        print(f"  {precision} {refinement}", flush=True)
+        alg = lsx.MI355XMixedLUFactorization(precision=precision, refinement=refinement, tol=1e-13, max_iters=100)
+        sol = lsx.solve(lsx.LinearProblem(A, b), alg)
+        st = sol.stats
+        print(f"  {precision}: {st}", flush=True)
+        assert sol.retcode == lsx.ReturnCode.Success, st
+        x = sol.u
+        assert berr(x) <= 1e-13
+        assert st["backward_error"] <= 1e-13
+        fwd = np.linalg.norm(x.cpu().numpy() - x64) / np.linalg.norm(x64)
+        print(f"  {precision}: forward error vs LAPACK {fwd:.2e}", flush=True)
+        assert fwd <= cond * 1e-12
+        if refinement == "gmres-ir":
+            assert st["mode_used"] == 1  # GMRES-IR, as the workload asks at cond 1e6
+    del A, Ah
+    torch.cuda.empty_cache()
