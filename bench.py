@@ -128,7 +128,7 @@ def timed(fn, reps):
     return best, out
 
 
-def run_mixed(lsx, _abi, ctx, L, n, conds, reps):
+def run_mixed(lsx, _abi, ctx, L, n, conds, reps, cpu=True):
     """Mixed-precision LU + FP64 refinement against plain FP64 LU on conditioned systems."""
     import torch
     P = lsx.problems
@@ -156,6 +156,19 @@ def run_mixed(lsx, _abi, ctx, L, n, conds, reps):
 
             t64, _ = timed(f64, reps)
             ent["fp64"] = dict(ms=round(t64 * 1e3, 3), backward_error=backward_error_t(A, x, b))
+            if cpu and cond == conds[0]:
+                # the reference's CPU path on the same system: LAPACK dgetrf + dgetrs, all BLAS threads
+                import scipy.linalg.lapack as lapack
+                Ah = np.asfortranarray(Acm.cpu().numpy().T)
+                bh = b.cpu().numpy()[:, 0].copy()
+                t0 = time.perf_counter()
+                lu, piv, info = lapack.dgetrf(Ah, overwrite_a=True)
+                lapack.dgetrs(lu, piv, bh)
+                tc = time.perf_counter() - t0
+                del Ah, lu
+                ent["cpu_lapack_fp64"] = dict(ms=round(tc * 1e3, 1), threads=int(_blas_cores()), kind="reference",
+                                              sample="one dgetrf + dgetrs of this system (scipy's OpenBLAS)")
+                log(f"  cond={cond:.0e} CPU LAPACK FP64: {tc * 1e3:.0f} ms")
             for name in ("fp32", "bf16"):
                 st = _abi.RefineStats()
 
@@ -317,6 +330,128 @@ def gmres_step_bytes(N, nnz, iters, memory):
     return conv, mini
 
 
+def _blas_cores():
+    try:
+        from threadpoolctl import threadpool_info
+        return max((i.get("num_threads", 1) for i in threadpool_info() if i.get("internal_api") == "openblas"),
+                   default=os.cpu_count())
+    except Exception:
+        return os.cpu_count()
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "?"
+
+
+def run_cpu_baselines(lsx, _abi, ctx, L, budget_s=12.0):
+    """The reference's CPU path on the host, per workload, with all BLAS threads and with one:
+    LAPACK dgetrf/dgetrs from scipy's OpenBLAS (the library LUFactorization calls; julia is not
+    installed) and, for SimpleGMRES, the oracle's C restatement of src/simplegmres.jl (one
+    thread).  Each figure is a bounded sample of its workload; the GPU figure beside it is the
+    same sample on the device."""
+    import torch
+    import scipy.linalg.lapack as lapack
+    from threadpoolctl import threadpool_limits
+    P = lsx.problems
+    cores = int(_blas_cores())
+    out = dict(cpu=_cpu_model(), blas_threads_all=cores)
+
+    def lapack_solve(A, b):
+        lu, piv, info = lapack.dgetrf(A)
+        x, _ = lapack.dgetrs(lu, piv, b)
+        return lu, piv, x
+
+    # dense LU, n = 1024 (A, b, b' iid U(0,1)): factor + solve, then a re-solve with the factors
+    n = 1024
+    A, b, b2 = P.dense_uniform(n)
+    A = np.asfortranarray(A)
+    ent = {}
+    for label, th in (("all_cores", cores), ("one_thread", 1)):
+        with threadpool_limits(limits=th, user_api="blas"):
+            lapack_solve(A, b)
+            reps, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 1.0:
+                lu, piv, x = lapack_solve(A, b)
+                reps += 1
+            tf = (time.perf_counter() - t0) / reps
+            reps, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < 0.5:
+                lapack.dgetrs(lu, piv, b2)
+                reps += 1
+            tr = (time.perf_counter() - t0) / reps
+        ent[label] = dict(threads=th, factor_solve_ms=round(tf * 1e3, 3), resolve_ms=round(tr * 1e3, 4))
+    h = ctypes.c_void_p()
+    ctx.check(L.lsx_lu_create(ctx.handle, n, _abi.LSX_F64, ctypes.byref(h)), "lsx_lu_create")
+    dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+    db, db2 = torch.from_numpy(b).cuda(), torch.from_numpy(b2).cuda()
+    dx = torch.empty_like(db)
+    torch.cuda.synchronize()
+
+    def gpu_fs():
+        ctx.check(L.lsx_lu_factor(h, dA.data_ptr(), n), "lsx_lu_factor")
+        ctx.check(L.lsx_lu_solve(h, 1, db.data_ptr(), n, dx.data_ptr(), n), "lsx_lu_solve")
+
+    def gpu_rs():
+        ctx.check(L.lsx_lu_solve(h, 1, db2.data_ptr(), n, dx.data_ptr(), n), "lsx_lu_solve")
+
+    tg, _ = timed(gpu_fs, 20)
+    tgr, _ = timed(gpu_rs, 50)
+    ipiv = np.empty(n, dtype=np.int64)
+    ctx.check(L.lsx_lu_get_factors(h, None, n, ipiv.ctypes.data), "lsx_lu_get_factors")
+    L.lsx_lu_destroy(h)
+    ent["gpu"] = dict(factor_solve_ms=round(tg * 1e3, 3), resolve_ms=round(tgr * 1e3, 4),
+                      pivots_bit_exact_vs_lapack=bool(np.array_equal(ipiv, piv + 1)))
+    out["dense_lu_n1024_uniform"] = ent
+    log(f"cpu baselines: n=1024 factor+solve CPU {ent['all_cores']['factor_solve_ms']} ms "
+        f"({cores} threads) / {ent['one_thread']['factor_solve_ms']} ms (1 thread), GPU {ent['gpu']['factor_solve_ms']} ms")
+
+    # plain FP64 LU at scale, one thread: dgetrf + dgetrs (16 RHS) at n = 8192
+    n = 8192
+    Ag, Bg = P.dense_gaussian(n, 16)
+    Ag = np.asfortranarray(Ag)
+    Bg = np.asfortranarray(Bg)
+    with threadpool_limits(limits=1, user_api="blas"):
+        t0 = time.perf_counter()
+        lu, piv, x = lapack_solve(Ag, Bg)
+        t1 = time.perf_counter() - t0
+    out["dense_lu_n8192_one_thread"] = dict(threads=1, seconds=round(t1, 3),
+                                            tflops=round((luflop(n) + 2.0 * n * n * 16) / t1 / 1e12, 4))
+    del Ag, Bg, lu, x
+    log(f"cpu baselines: n=8192 one thread {t1:.1f} s")
+
+    # batched: a per-system LAPACK loop over a sample of the 20,000-system distribution
+    sizes, Ap, bp = P.batched_systems(count=1000, seed=9)
+    offs = np.concatenate([[0], np.cumsum(sizes * sizes)[:-1]])
+    voffs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    blocks = [(np.asfortranarray(Ap[o:o + k * k].reshape(k, k, order="F")), bp[v:v + k])
+              for o, v, k in zip(offs, voffs, (int(k) for k in sizes))]
+    ent = {}
+    for label, th in (("all_cores", cores), ("one_thread", 1)):
+        with threadpool_limits(limits=th, user_api="blas"):
+            t0 = time.perf_counter()
+            done = 0
+            for Ai, bi in blocks:
+                lapack_solve(Ai, bi)
+                done += 1
+                if time.perf_counter() - t0 > budget_s:
+                    break
+            t = time.perf_counter() - t0
+        ent[label] = dict(threads=th, systems=done, solves_per_s=round(done / t, 1))
+    out["batched_lu_loop"] = dict(sample="1000 systems of the batched distribution (n uniform in [8, 512], "
+                                         "A = I - g J), one dgetrf + dgetrs per system", **ent)
+    log(f"cpu baselines: batched loop {ent['all_cores']['solves_per_s']} / {ent['one_thread']['solves_per_s']} solves/s")
+    del blocks, Ap
+
+    return out
+
+
 def run_gmres(lsx, _abi, ctx, L, m, pe, memory, maxiters, cpu_m):
     """One SimpleGMRES(memory) solve with the Jacobi preconditioner on the m x m convection-
     diffusion grid (inputs resident in HBM), and the same solver's CPU restatement (the oracle,
@@ -430,15 +565,18 @@ def run_gmres_batched(lsx, _abi, ctx, L, nblocks=4096, side=16, loop_sample=64):
 
 
 def _submit_synth(srv, req, gmres_maxiters):
+    """gmres_maxiters 0: SimpleGMRES runs to its stopping rule with the reference's default
+    maxiters = length(b) (src/common.jl:764)."""
     if req["kind"] == "dense":
         return srv.submit_dense_synth(req["n"], req["seed"])
-    return srv.submit_gmres_synth(req["m"], req["pe"], req["seed"], memory=30, reltol=1e-8, maxiters=gmres_maxiters)
+    return srv.submit_gmres_synth(req["m"], req["pe"], req["seed"], memory=30, reltol=1e-8,
+                                  maxiters=gmres_maxiters or None)
 
 
-def serving_loop_baseline(lsx, trace, gmres_maxiters, budget_gib, max_seconds=20.0):
+def serving_loop_baseline(lsx, trace, gmres_maxiters, budget_gib, max_seconds=30.0):
     """The per-problem loop the serving path is compared with: the same requests (the trace's
-    first ones, in order), each submitted only after the previous one completed, so nothing is
-    batched or overlapped; same kernels and device-side inputs."""
+    first ones, in order, for at most max_seconds), each submitted only after the previous one
+    completed, so nothing is batched or overlapped; same kernels and device-side inputs."""
     srv = lsx.Server(hbm_budget=int(budget_gib) << 30, nstreams=4)
     for req in trace[:3]:  # warm-up: lane handles and code objects
         t = _submit_synth(srv, req, gmres_maxiters)
@@ -458,7 +596,7 @@ def serving_loop_baseline(lsx, trace, gmres_maxiters, budget_gib, max_seconds=20
     return dict(requests=done, seconds=round(el, 3), req_per_s=round(done / el, 1))
 
 
-def run_serving(lsx, seconds, rate, gmres_maxiters, budget_gib):
+def run_serving(lsx, seconds, rate, gmres_maxiters, budget_gib, cpu=True):
     """Poisson arrivals of the serving mix on the synthetic (device-generated) request path."""
     P = lsx.problems
     trace = P.serving_trace(rate=rate, seconds=seconds)
@@ -481,30 +619,85 @@ def run_serving(lsx, seconds, rate, gmres_maxiters, budget_gib):
     srv.drain()
     t_end = srv.clock()
     lat = {"dense": [], "gmres": []}
-    conv = 0
-    for t, kind in pending:
+    conv = errors = iters = 0
+    steps = []  # (grid side, Arnoldi steps) of each served SimpleGMRES request
+    for (t, kind), req in zip(pending, trace):
         r = srv.poll(t)
+        if r["status"] != 0:  # no valid result (a wait gave up, bad input): not served
+            errors += 1
+            continue
         lat[kind].append(r["t_done"] - r["t_submit"])
-        if kind == "gmres" and r["retcode"] == lsx.ReturnCode.Success:
-            conv += 1
+        if kind == "gmres":
+            iters += int(r["iters"])
+            steps.append((req["m"], int(r["iters"])))
+            if r["retcode"] == lsx.ReturnCode.Success:
+                conv += 1
     st = srv.stats()
     srv.close()
     allv = np.array(lat["dense"] + lat["gmres"])
     q = lambda v, p: round(float(np.percentile(v, p)) * 1e3, 3) if len(v) else None
-    served = len(trace) / (t_end - c0)
-    log(f"serving: served {served:.1f} req/s; timing the one-at-a-time loop on the same trace")
+    served = len(allv) / (t_end - c0)
+    log(f"serving: served {served:.2f} req/s in {t_end - c0:.1f} s; timing the one-at-a-time loop on the same trace")
     loop = serving_loop_baseline(lsx, trace, gmres_maxiters, budget_gib)
-    res = dict(offered_req_per_s=rate, arrival_seconds=seconds, requests=len(trace),
-               served_req_per_s=round(served, 1),
+    res = dict(offered_req_per_s=rate, arrival_seconds=seconds, requests=len(trace), errors=errors,
+               makespan_s=round(t_end - c0, 3), served_req_per_s=round(served, 2),
                p50_ms=q(allv, 50), p99_ms=q(allv, 99),
                dense=dict(count=len(lat["dense"]), p50_ms=q(lat["dense"], 50), p99_ms=q(lat["dense"], 99)),
                gmres=dict(count=len(lat["gmres"]), p50_ms=q(lat["gmres"], 50), p99_ms=q(lat["gmres"], 99),
-                          converged=conv, maxiters=gmres_maxiters),
+                          converged=conv, arnoldi_steps=iters,
+                          maxiters=gmres_maxiters or "length(b) (reference default)"),
                batched_launches=st["batched_launches"], batched_requests=st["batched_requests"],
                peak_hbm_gib=round(st["peak_bytes"] / 2**30, 2),
                loop_baseline=loop, speedup_vs_loop=round(served / loop["req_per_s"], 2))
     log(f"serving: served {res['served_req_per_s']} req/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms; "
         f"loop {loop['req_per_s']} req/s -> {res['speedup_vs_loop']}x")
+    if cpu:
+        res["cpu_replay"] = serving_cpu_replay(lsx, trace, steps)
+    return res
+
+
+def serving_cpu_replay(lsx, trace, steps):
+    """The serving requests one at a time on the host: every dense request by LAPACK dgetrf +
+    dgetrs (all BLAS threads, then one), timed; every SimpleGMRES request at the oracle's cost
+    per Arnoldi step (the C restatement of src/simplegmres.jl, one thread; 60 steps timed per grid
+    size, the cost scaled by N between the timed sizes) times the steps the served solve took
+    (the same algorithm, so the same count within one)."""
+    import scipy.linalg.lapack as lapack
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    P = lsx.problems
+    cores = int(_blas_cores())
+    dense = [req for req in trace if req["kind"] == "dense"]
+    blocks = [P.dense_block(req["n"], req["seed"]) for req in dense]
+    out = {}
+    for label, th in (("all_cores", cores), ("one_thread", 1)):
+        with threadpool_limits(limits=th, user_api="blas"):
+            t0 = time.perf_counter()
+            for Ai, bi in blocks:
+                lu, piv, info = lapack.dgetrf(Ai)
+                lapack.dgetrs(lu, piv, bi)
+            out[label] = time.perf_counter() - t0
+    per_step = {}
+    for m in (64, 256, 1024):
+        rp, ci, v = P.convection_diffusion(m, 5.0)
+        N = m * m
+        b = P.rng_for(13, m).standard_normal(N)
+        t0 = time.perf_counter()
+        O.gmres(rp, ci, v, b, memory=30, restart=True, reltol=1e-30, abstol=0.0, maxiters=60, pl=np.full(N, 4.0))
+        per_step[m] = (time.perf_counter() - t0) / 60
+    ms_ = np.array(sorted(per_step))
+    cost = np.array([per_step[m] / (m * m) for m in ms_])  # seconds per step per row
+    gmres_s = sum(it * np.interp(m, ms_, cost) * m * m for m, it in steps)
+    nreq = len(dense) + len(steps)
+    res = dict(requests=nreq, gmres_seconds_estimated=round(gmres_s, 1),
+               gmres_step_ms={int(m): round(per_step[m] * 1e3, 3) for m in ms_})
+    for label in ("all_cores", "one_thread"):
+        res[label] = dict(dense_seconds=round(out[label], 3),
+                          req_per_s=round(nreq / (out[label] + gmres_s), 3))
+    res["threads_all"] = cores
+    log(f"serving: CPU replay {res['all_cores']['req_per_s']} req/s (dense {out['all_cores']:.2f} s measured, "
+        f"SimpleGMRES {gmres_s:.0f} s from the oracle's step cost)")
     return res
 
 
@@ -516,13 +709,16 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="mixed,gmres,batched,serving",
-                    help="comma list of extra sections beside the FP64 LU line (mixed, gmres, batched, serving; '' = none)")
+    ap.add_argument("--sections", default="mixed,gmres,batched,serving,cpu",
+                    help="comma list of extra sections beside the FP64 LU line (mixed, gmres, batched, serving, "
+                         "cpu; '' = none)")
     ap.add_argument("--mixed-n", type=int, default=16384)
     ap.add_argument("--batch-count", type=int, default=20000)
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=0.05,
+                    help="arrival window of the serving trace (every request is served to completion)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--serve-gmres-maxiters", type=int, default=300)
+    ap.add_argument("--serve-gmres-maxiters", type=int, default=0,
+                    help="0: length(b), the reference's default; SimpleGMRES runs to reltol 1e-8")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -679,13 +875,17 @@ def main():
     del G, Gb, A, B
     if "mixed" in sections:
         result["mixed"] = dict(n=args.mixed_n, nrhs=1, tol=1e-13,
-                               runs=run_mixed(lsx, _abi, ctx, L, args.mixed_n, (1e2, 1e4, 1e6), 2))
+                               runs=run_mixed(lsx, _abi, ctx, L, args.mixed_n, (1e2, 1e4, 1e6), 2,
+                                              cpu=not args.no_cpu_baseline))
     if "gmres" in sections:
         result["gmres"] = run_gmres(lsx, _abi, ctx, L, 1024, 5.0, 30, 300, 256)
     if "batched" in sections:
         result["batched"] = run_batched(lsx, _abi, ctx, L, args.batch_count, 3, 300)
     if "serving" in sections:
-        result["serving"] = run_serving(lsx, args.serve_seconds, args.serve_rate, args.serve_gmres_maxiters, 64)
+        result["serving"] = run_serving(lsx, args.serve_seconds, args.serve_rate, args.serve_gmres_maxiters, 64,
+                                        cpu=not args.no_cpu_baseline)
+    if "cpu" in sections and not args.no_cpu_baseline:
+        result["cpu_baselines"] = run_cpu_baselines(lsx, _abi, ctx, L)
     print(json.dumps(result), flush=True)
 
 
