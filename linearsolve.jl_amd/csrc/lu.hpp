@@ -56,6 +56,12 @@ struct lsx_lu {
     double* hrpart = nullptr;     // pinned
     // mixed precision state
     float* F32 = nullptr;         // low-precision working factors (FP32 storage)
+    // tournament-pivoting block factorization (calu.hpp) of the mixed path
+    uint64_t* calu_cand = nullptr;  // candidate records (flag-in-word)
+    uint64_t* calu_bc = nullptr;    // broadcast L11\U11, U12, pivot rows (flag-in-word)
+    uint32_t* calu_ctr = nullptr;   // arrival counter
+    uint32_t calu_cbase = 0;      // counter value the next launch starts from
+    uint32_t calu_seq = 1;        // sequence number of the next launch's first panel
     int64_t ld32 = 0;
     double* rwork = nullptr;      // residual / correction workspace (n x cap)
     void* aux = nullptr;          // mixed-path extra state

@@ -59,8 +59,15 @@ static void launch_laswp_t(lsx_lu* lu, T* F, int64_t ld, int64_t k0, int kb, con
 // lu->stream, then block b + 1 is factored on lu->stream2 while lu->stream updates the remaining
 // columns.  The row interchanges of a block are applied right of it at once and, when a column
 // segment fits in LDS, left of it (the L columns) in one pass at the end.
-template <typename T, typename GU, typename GT>
-int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT gemm_trsm) {
+// The exact panels of a block (below) unless block_fact(jo, jbo) factors the whole block column
+// itself, rows in place, and writes its pivots into lu->ipiv (the mixed path's tournament
+// pivoting); then the block's interchanges are applied to its own columns here.
+struct NoBlockFact {
+    bool operator()(int64_t, int) const { return false; }
+};
+
+template <typename T, typename GU, typename GT, typename BF = NoBlockFact>
+int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT gemm_trsm, BF block_fact = BF()) {
     lsx_ctx* ctx = lu->ctx;
     const int64_t n = lu->n;
     const hipStream_t s0 = lu->stream_upd ? lu->stream_upd : lu->stream, s1 = lu->stream2;
@@ -83,6 +90,22 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
     // their interchanges and updates inside the block, diagonal-block inverses, the block's
     // interchange list (parity par) and the row map
     auto panel_block = [&](int64_t jo, int jbo, int par) {
+        if (block_fact(jo, jbo)) {
+            // rows were factored in place: the block's interchanges on its own columns, then the
+            // inverses of its diagonal blocks (in the final row order)
+            hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
+                               lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);
+            launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, jo,
+                              jo + jbo, 0);
+            const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
+            const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
+            hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(DB), 0, lu->cur, F, ld, n, b0, DL, DU);
+            lu->prof.end(lu->cur, pi);
+            if (defer)
+                hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
+                                   lu->pinv + (jo / NBO) * n, n);
+            return;
+        }
         for (int64_t ji = jo; ji < jo + jbo; ji += PW) {
             const int W = (int)std::min<int64_t>(PW, jo + jbo - ji);
             launch_panel_t<T>(lu, F, ld, ji, W);

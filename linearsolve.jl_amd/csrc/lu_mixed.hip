@@ -13,6 +13,8 @@
 #include <vector>
 
 #include "gemm.hpp"
+#include "gemm_split.hpp"
+#include "calu.hpp"
 #include "lu_driver.hpp"
 #include "residual.hpp"
 
@@ -62,109 +64,6 @@ __global__ void k_colmax_scale(const float* __restrict U, int64_t ld, int K, int
     int e = 0;
     if (m > 0.f && isfinite(m)) (void)frexpf(m, &e);
     out[j] = ldexpf(1.f, -e);
-}
-
-constexpr int MBM = 128, MBN = 128;
-
-__device__ __forceinline__ __bf16 to_bf16(float f) {
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    const uint16_t h = (uint16_t)(u >> 16);
-    return __builtin_bit_cast(__bf16, h);
-}
-
-// C -= A B with BF16 operands and FP32 accumulation (v_mfma_f32_16x16x32_bf16).  Column j of B
-// (the U12 block) is multiplied by its power-of-two scale[j] before rounding and column j of the
-// product by 1/scale[j] after accumulation (both exact).  A (L21) holds multipliers of magnitude
-// <= 1.
-__global__ __launch_bounds__(256, 2) void k_gemm_bf16(const float* __restrict Ag, int64_t lda, const float* Bg,
-                                                   int64_t ldb, float* Cg, int64_t ldc, int M, int N, int K,
-                                                   const float* __restrict scale) {
-    constexpr int BK = 32, PAD = 8;
-    const int TM = (M + MBM - 1) / MBM, TN = (N + MBN - 1) / MBN, T = TM * TN;
-    const int per = (T + 7) / 8;
-    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (tile >= T) return;
-    const int row0 = (tile % TM) * MBM, col0 = (tile / TM) * MBN;
-    __shared__ __align__(16) __bf16 As[2][MBM][BK + PAD];   // As[row][k]
-    __shared__ __align__(16) __bf16 Bs[2][MBN][BK + PAD];   // Bs[col][k]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    v4f acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    float ra[16], rb[16];
-    auto gload = [&](int k0) {
-#pragma unroll
-        for (int it = 0; it < 16; ++it) {
-            const int idx = it * 256 + tid;
-            const int r = idx & (MBM - 1), k = idx >> 7;          // A: rows fastest (coalesced)
-            ra[it] = (row0 + r < M && k0 + k < K) ? Ag[(int64_t)(k0 + k) * lda + row0 + r] : 0.f;
-            const int kb = idx & (BK - 1), c = idx >> 5;          // B: k fastest (coalesced)
-            rb[it] = (col0 + c < N && k0 + kb < K) ? Bg[(int64_t)(col0 + c) * ldb + k0 + kb] * scale[col0 + c] : 0.f;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int it = 0; it < 16; ++it) {
-            const int idx = it * 256 + tid;
-            As[buf][idx & (MBM - 1)][idx >> 7] = to_bf16(ra[it]);
-            Bs[buf][idx >> 5][idx & (BK - 1)] = to_bf16(rb[it]);
-        }
-    };
-    const int nk = (K + BK - 1) / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int st = 0; st < nk; ++st) {
-        const int buf = st & 1;
-        if (st + 1 < nk) gload((st + 1) * BK);
-        const int kq = 8 * (lane >> 4);
-        v8bf x[4], y[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) x[i] = *reinterpret_cast<const v8bf*>(&Bs[buf][wn * 64 + i * 16 + (lane & 15)][kq]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) y[j] = *reinterpret_cast<const v8bf*>(&As[buf][wm * 64 + j * 16 + (lane & 15)][kq]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[i], y[j], acc[i][j], 0, 0, 0);
-        if (st + 1 < nk) {
-            sstore(buf ^ 1);
-            __syncthreads();
-        }
-    }
-    // sixteen C values at a time are loaded before any of them is stored
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float cv[4][4], inv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
-            inv[q] = col < N ? 1.f / scale[col] : 0.f;
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
-                cv[j][q] = (row < M && col < N) ? Cg[(int64_t)col * ldc + row] : 0.f;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int row = row0 + wm * 64 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = col0 + wn * 64 + i * 16 + 4 * (lane >> 4) + q;
-                if (row < M && col < N) Cg[(int64_t)col * ldc + row] = cv[j][q] - acc[i][j][q] * inv[q];
-            }
-        }
-    }
 }
 
 // Y += alpha X, column-major n x nrhs.
@@ -225,6 +124,16 @@ int mixed_alloc(lsx_lu* lu) {
     const int64_t pw = std::max<int64_t>(resid_part_words(n), cdiv(n, 256) * 64);
     LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * pw));
     LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * pw, hipHostMallocDefault));
+    // tournament pivoting: candidate records, broadcast, arrival counter
+    LSX_HIP(ctx, hipMalloc(&lu->calu_cand, sizeof(uint64_t) * CALU_PMAX * CW * CREC));
+    LSX_HIP(ctx, hipMalloc(&lu->calu_bc, sizeof(uint64_t) * CBC));
+    LSX_HIP(ctx, hipMalloc(&lu->calu_ctr, 256));
+    LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CALU_PMAX * CW * CREC, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->calu_bc, 0, sizeof(uint64_t) * CBC, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->calu_ctr, 0, 256, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+    lu->calu_cbase = 0;
+    lu->calu_seq = 1;
     return LSX_OK;
 }
 
@@ -246,6 +155,11 @@ void mixed_free(lsx_lu* lu) {
     (void)hipFree(m->V);
     (void)hipFree(m->w);
     (void)hipFree(m->hvec);
+    (void)hipFree(lu->calu_cand);
+    (void)hipFree(lu->calu_bc);
+    (void)hipFree(lu->calu_ctr);
+    lu->calu_cand = lu->calu_bc = nullptr;
+    lu->calu_ctr = nullptr;
     delete m;
     lu->aux = nullptr;
 }
@@ -259,16 +173,53 @@ static void gemm_f32(lsx_lu* lu, const float* A, int64_t lda, const float* B, in
     lu->prof.end(lu->cur, pi);
 }
 
-static void gemm_bf16(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+// Trailing update of the mixed factorization on BF16 MFMA: FP32 factors with products of three
+// BF16 pieces per operand (FP32 accuracy), or BF16 factors (one piece, per-column power-of-two
+// scales of U12).
+static void gemm_lowp(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                       int64_t M, int64_t N, int64_t K) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    float* sc = ms(lu)->scale + (lu->cur == lu->stream2 ? lu->n + 64 : 0);  // one set per getrf stream
-    hipLaunchKernelGGL(k_colmax_scale, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
-    const int64_t T = cdiv(M, MBM) * cdiv(N, MBN);
-    const int pi = lu->prof.begin(lu->cur, PK_GEMM, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
-    hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)(8 * cdiv(T, 8))), dim3(256), 0, lu->cur, A, lda, B, ldb, C,
-                       ldc, (int)M, (int)N, (int)K, (const float*)sc);
+    const bool bf16 = lu->dtype == LSX_BF16;
+    float* sc = nullptr;
+    if (bf16) {
+        sc = ms(lu)->scale + (lu->cur == lu->stream2 ? lu->n + 64 : 0);  // one set per getrf stream
+        hipLaunchKernelGGL(k_colmax_scale, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
+    }
+    const bool big = cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)lu->ctx->num_cus;
+    const int pi = lu->prof.begin(lu->cur, big ? PK_GEMM : PK_GEMM_SMALL, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
+    if (bf16) launch_gemm_split<1>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K, sc);
+    else launch_gemm_split<3>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K, nullptr);
     lu->prof.end(lu->cur, pi);
+}
+
+// Block column [jo, jo + jbo) by tournament pivoting (k_block_calu) on lu->cur; false when the
+// block has more rows than one launch holds (the exact panels take it then).
+static bool calu_block(lsx_lu* lu, int64_t jo, int jbo) {
+    const int64_t m = lu->n - jo;
+    const int P = (int)cdiv(m, CBS);
+    if (P > CALU_PMAX) return false;
+    const int np = (int)cdiv(jbo, CW);
+    CaluArgs a;
+    a.A = lu->F32;
+    a.ld = lu->ld32;
+    a.n = lu->n;
+    a.jo = jo;
+    a.JB = jbo;
+    a.P = P;
+    a.ipiv = lu->ipiv;
+    a.dinfo = lu->dinfo;
+    a.cand = lu->calu_cand;
+    a.bc = lu->calu_bc;
+    a.ctr = lu->calu_ctr;
+    a.cbase = lu->calu_cbase;
+    a.seq = lu->calu_seq;
+    a.timeout = wait_ticks(lu->ctx, 5.0);
+    lu->calu_cbase += (uint32_t)(np * P);
+    lu->calu_seq += (uint32_t)np;
+    const int pi = lu->prof.begin(lu->cur, PK_PANEL, (double)m * jbo * jbo, 2.0 * sizeof(float) * (double)m * jbo);
+    hipLaunchKernelGGL(k_block_calu, dim3(P), dim3(CBS), 0, lu->cur, a);
+    lu->prof.end(lu->cur, pi);
+    return true;
 }
 
 int mixed_getrf(lsx_lu* lu) {
@@ -285,14 +236,17 @@ int mixed_getrf(lsx_lu* lu) {
     lu->prof.end(lu->stream, pc);
     auto gt = [lu](const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc, int64_t M,
                    int64_t N, int64_t K, int mode) { gemm_f32(lu, A, lda, B, ldb, C, ldc, M, N, K, mode); };
-    int rc;
-    if (lu->dtype == LSX_BF16) {
-        auto gu = [lu](const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc, int64_t M,
-                       int64_t N, int64_t K, int) { gemm_bf16(lu, A, lda, B, ldb, C, ldc, M, N, K); };
-        rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gu, gt);
-    } else {
-        rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gt, gt);
+    auto gu = [lu](const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc, int64_t M,
+                   int64_t N, int64_t K, int) { gemm_lowp(lu, A, lda, B, ldb, C, ldc, M, N, K); };
+    if (lu->calu_seq > 0xF0000000u || lu->calu_cbase > 0xF0000000u) {
+        LSX_HIP(ctx, hipMemsetAsync(lu->calu_ctr, 0, 256, lu->stream));
+        LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CALU_PMAX * CW * CREC, lu->stream));
+        LSX_HIP(ctx, hipMemsetAsync(lu->calu_bc, 0, sizeof(uint64_t) * CBC, lu->stream));
+        lu->calu_cbase = 0;
+        lu->calu_seq = 1;
     }
+    auto bf = [lu](int64_t jo, int jbo) { return calu_block(lu, jo, jbo); };
+    const int rc = getrf_blocked<float>(lu, lu->F32, lu->ld32, m->DL, m->DU, gu, gt, bf);
     double an = 0.0;
     for (int64_t i = 0; i < nb; ++i) an = nanmax(an, m->hpart[i]);
     m->anorm_h = an;

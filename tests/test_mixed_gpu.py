@@ -7,6 +7,7 @@ algorithms (OpenBLAS32MixedLUFactorization, src/openblas.jl:461-543) stop at ~1e
 """
 import numpy as np
 import pytest
+import scipy.linalg.lapack as lapack
 
 from conftest import backward_error
 
@@ -134,3 +135,52 @@ def test_conditioned_full_size(lsx, cond):
             assert st["mode_used"] == 1  # GMRES-IR, as the workload asks at cond 1e6
     del A, Ah
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("n", [200, 1000, 2500])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_factor_scaled_residual(lsx, precision, n):
+    """The low-precision factors satisfy the scaled-residual test of the reference's
+    test/Core/blocked_lufact.jl:11-28, ||PA - LU||_1 / (||L||_1 ||U||_1 n u) < 20, with u the unit
+    roundoff the factorization promises: FP32's 2^-24 for FP32 factors (their trailing products
+    come from three BF16 pieces per operand, which must stay below that), BF16's 2^-8 for BF16
+    factors.  A dropped or mis-scaled piece would leave FP32 factors at about 2^-16."""
+    rng = np.random.default_rng(n + (0 if precision == "fp32" else 1))
+    A = rng.standard_normal((n, n))
+    dt = lsx.LSX_F32 if precision == "fp32" else lsx.LSX_BF16
+    h = lsx._LUHandle(lsx.Context.get(0), n, dt)
+    assert h.factor(A) == 0
+    F, ipiv = h.factors()
+    Lf = np.tril(F, -1) + np.eye(n)
+    Uf = np.triu(F)
+    PA = A.copy()
+    for k in range(n):  # LAPACK swap sequence
+        p = ipiv[k] - 1
+        if p != k:
+            PA[[k, p], :] = PA[[p, k], :]
+    u = 2.0 ** -24 if precision == "fp32" else 2.0 ** -8
+    res = np.abs(PA - Lf @ Uf).sum(axis=0).max() / (np.abs(Lf).sum(axis=0).max() * np.abs(Uf).sum(axis=0).max() * n * u)
+    # and against LAPACK's FP32 sgetrf of the same matrix: the relative factorization error
+    # ||PA - LU||_F / ||A||_F of FP32 factors must be of sgetrf's size (products of two BF16 pieces
+    # per operand instead of three would leave it about 2^7 times larger)
+    e_dev = np.linalg.norm(PA - Lf @ Uf) / np.linalg.norm(A)
+    lu32, piv32, info32 = lapack.sgetrf(A.astype(np.float32))
+    P32 = A.copy()
+    for k in range(n):
+        p = piv32[k]
+        if p != k:
+            P32[[k, p], :] = P32[[p, k], :]
+    lu32 = lu32.astype(np.float64)
+    e_ref = np.linalg.norm(P32 - (np.tril(lu32, -1) + np.eye(n)) @ np.triu(lu32)) / np.linalg.norm(A)
+    print(f"{precision} n={n}: scaled residual {res:.3e}, ||PA - LU||_F / ||A||_F {e_dev:.3e} "
+          f"(LAPACK sgetrf {e_ref:.3e})", flush=True)
+    assert res < 20.0
+    if precision == "fp32":
+        assert e_dev <= 4.0 * e_ref
+    # the pivots form a permutation (every row chosen once)
+    perm = np.arange(n)
+    for k in range(n):
+        p = ipiv[k] - 1
+        assert k <= p < n
+        perm[[k, p]] = perm[[p, k]]
+    assert sorted(perm) == list(range(n))
