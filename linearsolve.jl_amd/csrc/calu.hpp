@@ -31,6 +31,8 @@
 // of the columns right of it.  A zero pivot (all candidates zero in a column) takes the candidate
 // with the smallest row index, its multipliers are left zero, and info = first such column.
 #pragma once
+#include <type_traits>
+
 #include "lu_kernels.hpp"
 
 namespace lsx {
@@ -40,7 +42,9 @@ constexpr int CALU_PMAX = 32;   // workgroups: blocks of up to 16384 rows
 constexpr int CW = 32;          // inner panel width
 constexpr int CREM = NBO - CW;  // columns of a block right of its first inner panel
 
-constexpr int CREC = 1 + CW + CREM;           // words of a candidate record: label, panel, rest
+constexpr int CSL = CALU_PMAX * CW;           // candidate slots
+constexpr int CREMW = (1 + CW) * CSL;         // first word of the candidates' remaining columns
+constexpr int CCAND = CREMW + CSL * CREM;     // words of the candidate records
 constexpr int CBC = CW * CW + CW * CREM + CW;   // words of the broadcast: L11\U11, U12, pivot rows
 
 struct CaluArgs {
@@ -52,7 +56,7 @@ struct CaluArgs {
     int P;
     int64_t* ipiv;     // LAPACK swap sequence of the block, 1-based
     int64_t* dinfo;    // [0] info, [1] wait error flags
-    uint64_t* cand;    // candidate records [CALU_PMAX][CW][CREC] (label -1: none)
+    uint64_t* cand;    // candidate records, CCAND words (label -1: none)
     uint64_t* bc;      // broadcast [CBC]
     uint32_t* ctr;     // arrival counter
     uint32_t cbase;    // counter value when this launch starts
@@ -60,11 +64,16 @@ struct CaluArgs {
     int64_t timeout;
 };
 
-// 32-bit pivot key of an FP32 value: |x| as its (monotone) bit pattern + 1, NaN counted as 0;
-// 0 = not a candidate.  Half the shuffle work of the 64-bit key of the FP64 panel.
-__device__ __forceinline__ uint32_t pivot_key32(float x) {
+// Packed pivot key of a candidate with index idx < 2^IB: |x|'s bit pattern (monotone in |x|, NaN
+// counted as 0) with its IB low bits replaced by the complement of idx, plus one unit of the kept
+// bits so that every candidate stays above 0 = none.  One unsigned max is then the argmax: the
+// largest magnitude to a relative 2^(IB-23) (so every multiplier stays below 1 + 2^(IB-22) in
+// magnitude) and, among equal kept bits, the smallest index.
+template <int IB>
+__device__ __forceinline__ uint32_t calu_key(float x, uint32_t idx) {
     const float t = fabsf(x);
-    return (t == t ? __float_as_uint(t) : 0u) + 1u;
+    const uint32_t b = t == t ? __float_as_uint(t) : 0u;
+    return (((b >> IB) + 1u) << IB) | ((1u << IB) - 1u - idx);
 }
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     uint32_t y;
@@ -76,52 +85,40 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
     y = xrow32(x);       x = y > x ? y : x;
     return x;
 }
-// Lane holding the best (key, lab) pair of the wave: largest key, then smallest label (uniform).
-__device__ __forceinline__ int wave_argmax32(uint32_t u, int lab) {
-    const uint32_t m = wave_max_u32(u);
-    const uint64_t tie = __ballot(u == m);
-    if (__popcll(tie) == 1 || m == 0) return __ffsll((unsigned long long)tie) - 1;
-    const int mn = wave_min_i32(u == m ? lab : INT_MAX);
-    return __ffsll((unsigned long long)__ballot(u == m && lab == mn)) - 1;
+// f(integral_constant<int, I>) for I = 0 .. N-1, each call with its index a compile-time constant
+// (a loop whose steps index register arrays, too long for the unroller).
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
 }
 
-// Workgroup argmax of (key, label) over its 8 waves: each wave's best writes its key, label and
-// NV values to s[par][wave]; returns the winning wave (uniform).  key 0 = no candidate.
-// The values are those of v0, or of v1 where sel is set.
-template <int NV>
-__device__ __forceinline__ int calu_wg_argmax(uint32_t key, int lab, const float (&v0)[NV], const float (&v1)[NV],
-                                              bool sel, float (*s)[4 + NV], int lane, int wave) {
-    const int wl = wave_argmax32(key, lab);
-    if (lane == wl) {
-        float* d = s[wave];
-        reinterpret_cast<uint32_t*>(d)[0] = key;
-        reinterpret_cast<int*>(d)[2] = lab;
+// Values C .. CW-1 of a 16-byte aligned row in LDS, read as the 16-byte words that hold them.
+template <int C>
+__device__ __forceinline__ void load_row(const float* row, float (&r)[CW]) {
+    float4 t[CW / 4];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) d[4 + q] = sel ? v1[q] : v0[q];
+    for (int k = C / 4; k < CW / 4; ++k) t[k] = reinterpret_cast<const float4*>(row)[k];
+#pragma unroll
+    for (int k = 0; k < CW / 4; ++k) {
+        if (k < C / 4) {
+            r[4 * k] = r[4 * k + 1] = r[4 * k + 2] = r[4 * k + 3] = 0.f;
+        } else {
+            r[4 * k] = t[k].x;
+            r[4 * k + 1] = t[k].y;
+            r[4 * k + 2] = t[k].z;
+            r[4 * k + 3] = t[k].w;
+        }
     }
-    __syncthreads();
-    const uint32_t cu = lane < CBS / 64 ? reinterpret_cast<const uint32_t*>(s[lane])[0] : 0u;
-    const int cl = lane < CBS / 64 ? reinterpret_cast<const int*>(s[lane])[2] : INT_MAX;
-    return wave_argmax32(cu, cl);
 }
 
-// The pivot row's values are all read first (unconditionally, so the reads overlap: pivot_row)
-// and the update selects instead of branching: a test per slot around its read would make the
-// compiler wait for each read in turn.
-__device__ __forceinline__ void pivot_row(const float* prow, float (&pr)[CW - 1]) {
-#pragma unroll
-    for (int q = 0; q < CW - 1; ++q) pr[q] = prow[q + 1];
-}
-__device__ __forceinline__ void calu_step(float (&v)[CW], const float (&pr)[CW - 1], float pv, bool upd, int c) {
-    const float l = (upd && pv != 0.f) ? v[0] * (1.f / pv) : 0.f;
-    const int lim = upd ? CW - 1 - c : 0;
-#pragma unroll
-    for (int q = 0; q < CW - 1; ++q) {
-        const float nv = v[q + 1];
-        const float u = fmaf(-l, pr[q], nv);
-        v[q] = q < lim ? u : nv;
-    }
-    v[CW - 1] = l;
+// The largest of the eight waves' keys (two 16-byte LDS reads, the same address in every lane).
+__device__ __forceinline__ uint32_t max8_u32(const uint32_t* s) {
+    const uint4 a = *reinterpret_cast<const uint4*>(s);
+    const uint4 b = *reinterpret_cast<const uint4*>(s + 4);
+    return max(max(max(a.x, a.y), max(a.z, a.w)), max(max(b.x, b.y), max(b.z, b.w)));
 }
 
 // Payloads of the flag-in-word messages addr(i) selected by mask (i < N <= 64) once they carry
@@ -165,11 +162,14 @@ __device__ __forceinline__ bool calu_poll(F addr, uint64_t mask, uint32_t seq, u
 
 // The broadcast in LDS: packed L11\U11 of the panel and U12 of its pivot rows (namespace scope,
 // so that the row update below, a function of its own, reads them as LDS).
-__shared__ float s_LU[CW][CW + 1];
+// Rows of s_LU are 16-byte aligned, so that a row's values are read four at a time; s_dinv holds
+// the reciprocals of U11's diagonal (0 for a zero pivot).
+__shared__ __align__(16) float s_LU[CW][CW + 4];
 __shared__ __align__(16) float s_U12[CW][CREM + 4];
+__shared__ float s_dinv[CW];
 
 // A live row that is not a pivot row: l U11 = a (row vector, U11 upper with its diagonal),
-// column by column, then a_rest -= l U12.  pc0: the panel's first column from the block's first
+// column by column (times the diagonal's reciprocals), then a_rest -= l U12.  pc0: the panel's first column from the block's first
 // row; rob: the row's byte offset in a column.  FULL: the panel has all CW columns (every panel
 // but a last partial one); only then are the multipliers stored without a test per column (a
 // test per column makes the compiler keep every address and value live).
@@ -186,8 +186,7 @@ __device__ __forceinline__ void calu_row_update(float* pc0, int64_t ld, uint32_t
     }
 #pragma unroll
     for (int i = 0; i < CW; ++i) {
-        const float d = s_LU[i][i];
-        l[i] = ((FULL || i < W) && d != 0.f) ? l[i] / d : 0.f;
+        l[i] = (FULL || i < W) ? l[i] * s_dinv[i] : 0.f;
 #pragma unroll
         for (int c = i + 1; c < CW; ++c) l[c] = fmaf(-l[i], s_LU[i][c], l[c]);
     }
@@ -240,11 +239,18 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
     const uint32_t ro = valid ? (uint32_t)(r - a.jo) : 0u;
     const uint32_t rob = ro * 4u;  // in bytes (rows < 2^30)
     bool dead = false;  // a pivot row of an earlier inner panel of this block
-    __shared__ __align__(16) float s_win[2][CBS / 64][4 + CW];
+    // one elimination step's exchange between the waves (two sets, used in turn, so that one
+    // barrier per step suffices): each wave's largest key and that candidate's row and label
+    __shared__ __align__(16) uint32_t s_key[2][CBS / 64];
+    __shared__ __align__(16) float s_row[2][CBS / 64][CW];
+    __shared__ int s_rlab[2][CBS / 64];
+    __shared__ __align__(16) float s_zrow[CW];  // a zero column's pivot row (16-byte aligned)
+    __shared__ int s_zaux[CBS / 64 + 2];
     __shared__ int s_lab[CW];
     __shared__ int s_slot[CW];       // deciding workgroup: candidate slot of each pivot
+    __shared__ int s_pick[CW];       // this workgroup's candidates, in order: their threads
     __shared__ int s_allab[NBO];     // the block's pivot rows, panel after panel
-    __shared__ int s_final, s_count, s_abort;
+    __shared__ int s_final, s_abort;
     const int np = (a.JB + CW - 1) / CW;
     const int64_t ld = a.ld;
     float* A = a.A;
@@ -267,7 +273,9 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
             return *reinterpret_cast<float*>(reinterpret_cast<char*>(pc0 + (int64_t)c * ldp) + rob);
         };
 
-        // 1. local tournament: partial pivoting on this workgroup's live rows, then its picks
+        // 1. local tournament: partial pivoting on this workgroup's live rows, then its picks.
+        // The steps are unrolled (the column index is a constant in each), and only the multipliers'
+        // choices matter here, so a row's panel values are updated in place with no multipliers kept.
         {
             float v[CW];
 #pragma unroll
@@ -277,57 +285,89 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
             }
             bool live = valid && !dead;
             int myrank = -1;
-            if (tid == 0) s_count = 0;
-#pragma unroll 1
-            for (int c = 0; c < W; ++c) {
-                const int par = c & 1;
-                const uint32_t key = live ? pivot_key32(v[0]) : 0u;
-                const int wi = calu_wg_argmax<CW>(key, (int)r, v, v, false, s_win[par], lane, wave);
-                const float* win = s_win[par][wi];
-                if (reinterpret_cast<const uint32_t*>(win)[0] == 0u) break;  // no live row left (uniform)
-                const int glab = reinterpret_cast<const int*>(win)[2];
-                const float pv = win[4];
-                if (live && glab == (int)r) {
+            int cnt = W;  // candidates this workgroup offers (uniform)
+            bool done = false;
+            static_for<0, CW>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                if (done || c >= W) return;
+                constexpr int par = c & 1;
+                const uint32_t key = live ? calu_key<9>(v[c], (uint32_t)tid) : 0u;
+                const uint32_t m = wave_max_u32(key);
+                if (lane == 0) s_key[par][wave] = m;
+                if (key == m && m != 0u) {
+#pragma unroll
+                    for (int q = 0; q < CW; q += 4)
+                        *reinterpret_cast<float4*>(&s_row[par][wave][q]) = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+                }
+                __syncthreads();
+                const uint32_t M = max8_u32(s_key[par]);
+                if (M == 0u) {  // no live row left (uniform)
+                    cnt = c;
+                    done = true;
+                    return;
+                }
+                const int wt = (CBS - 1) - (int)(M & (CBS - 1));
+                // the pivot row, 16 bytes per read, all reads issued before the first use
+                float pr[CW];
+                load_row<c>(s_row[par][wt >> 6], pr);
+                if (tid == wt) {
                     myrank = c;
                     live = false;
                 }
-                float pr[CW - 1];
-                pivot_row(win + 4, pr);
-                calu_step(v, pr, pv, live, c);
-                if (tid == 0) s_count = c + 1;
-            }
+                const float pv = pr[c];
+                const float l = (live && pv != 0.f) ? v[c] * __builtin_amdgcn_rcpf(pv) : 0.f;
+#pragma unroll
+                for (int q = c + 1; q < CW; ++q) v[q] = fmaf(-l, pr[q], v[q]);
+            });
+            // arrival first: the records are read by their sequence numbers, not by this count
+            uint32_t arrived = 0;
+            if (tid == 0) arrived = atomicAdd(a.ctr, 1u);
+            if (myrank >= 0) s_pick[myrank] = tid;
             __syncthreads();
-            const int cnt = s_count;
-            // record of slot k: [0] row, [1, 33) panel values before this panel, [33, 33 + rem)
-            // the row's values in the block's remaining columns
-            uint64_t* recs = candp + (size_t)w * CW * CREC;
-            if (myrank >= 0) {
-                uint64_t* rec = recs + (size_t)myrank * CREC;
-                st_relaxed_u64(rec, ll_pack((uint32_t)r, seq));
+            // the candidates of slots w CW + k, copied by the whole workgroup (every load of a
+            // thread in flight together): words q = 0 (row) and 1 .. CW (panel values before this
+            // panel) at q CSL + slot, so that the deciding workgroup reads them a slot per lane;
+            // values in the block's remaining columns at CREMW + slot CREM + j (label -1: none)
+            {
+                constexpr int NPW = ((1 + CW) * CW + CBS - 1) / CBS;
+                float x[NPW];
 #pragma unroll
-                for (int q = 0; q < CW; ++q) {
-                    const float x = at(min(q, W - 1));
-                    st_relaxed_u64(rec + 1 + q, ll_pack(__float_as_uint(q < W ? x : 0.f), seq));
+                for (int i = 0; i < NPW; ++i) {
+                    const int idx = tid + i * CBS, q = idx / CW, k = idx % CW;
+                    const int pick = k < cnt ? s_pick[k] : 0;
+                    const int qc = min(max(q - 1, 0), W - 1);
+                    x[i] = (idx < (1 + CW) * CW && q >= 1 && q - 1 < W && k < cnt)
+                               ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(pc0 + (int64_t)qc * ldp) + 4u * (uint32_t)pick + 4u * (uint32_t)(w * CBS))
+                               : 0.f;
                 }
-                for (int j0 = 0; j0 < rem; j0 += 16) {  // 16 loads in flight, then their stores
-                    float x[16];
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) x[e] = at(W + min(j0 + e, rem - 1));
+                for (int i = 0; i < NPW; ++i) {
+                    const int idx = tid + i * CBS, q = idx / CW, k = idx % CW;
+                    if (idx < (1 + CW) * CW) {
+                        const uint32_t pl = q == 0 ? (k < cnt ? (uint32_t)(a.jo + (int64_t)w * CBS + s_pick[k]) : 0xFFFFFFFFu)
+                                                   : __float_as_uint(x[i]);
+                        st_relaxed_u64(candp + (size_t)q * CSL + w * CW + k, ll_pack(pl, seq));
+                    }
+                }
+                constexpr int NRW = (CW * CREM + CBS - 1) / CBS;
+                float y[NRW];
 #pragma unroll
-                    for (int e = 0; e < 16; ++e)
-                        if (j0 + e < rem) st_relaxed_u64(rec + 1 + CW + j0 + e, ll_pack(__float_as_uint(x[e]), seq));
+                for (int i = 0; i < NRW; ++i) {
+                    const int idx = tid + i * CBS, k = idx / CREM, j = idx % CREM;
+                    const int pick = k < cnt ? s_pick[k] : 0;
+                    const int col = rem > 0 ? W + min(j, rem - 1) : W - 1;  // inside the block
+                    y[i] = (k < cnt && j < rem)
+                               ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(pc0 + (int64_t)col * ldp) + 4u * (uint32_t)pick + 4u * (uint32_t)(w * CBS))
+                               : 0.f;
+                }
+#pragma unroll
+                for (int i = 0; i < NRW; ++i) {
+                    const int idx = tid + i * CBS, k = idx / CREM, j = idx % CREM;
+                    if (k < cnt && j < rem)
+                        st_relaxed_u64(candp + CREMW + (size_t)(w * CW + k) * CREM + j, ll_pack(__float_as_uint(y[i]), seq));
                 }
             }
-            if (tid >= cnt && tid < CW) {  // no candidate in this slot
-                uint64_t* rec = recs + (size_t)tid * CREC;
-                st_relaxed_u64(rec, ll_pack(0xFFFFFFFFu, seq));
-#pragma unroll
-                for (int q = 0; q < CW; ++q) st_relaxed_u64(rec + 1 + q, ll_pack(0u, seq));
-            }
-            if (tid == 0) {
-                const uint32_t old = atomicAdd(a.ctr, 1u);
-                s_final = (old == a.cbase + (uint32_t)(p * a.P + a.P - 1)) ? 1 : 0;
-            }
+            if (tid == 0) s_final = (arrived == a.cbase + (uint32_t)(p * a.P + a.P - 1)) ? 1 : 0;
             __syncthreads();
         }
 
@@ -335,19 +375,18 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
         const bool fin = s_final != 0;
         if (fin) {
             float f[2][CW];
-            int fl[2], fs[2];
+            int fl[2];
             bool fa[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int sl = tid + h * CBS;
-                fs[h] = sl;
                 fl[h] = -1;
 #pragma unroll
                 for (int q = 0; q < CW; ++q) f[h][q] = 0.f;
                 if (sl / CW < a.P) {
                     uint32_t x[1 + CW];
-                    const uint64_t* rec = candp + (size_t)sl * CREC;
-                    if (calu_poll<1 + CW>([&](int i) { return rec + i; }, (1ull << (1 + CW)) - 1, seq, x, a, &s_abort)) {
+                    if (calu_poll<1 + CW>([&](int i) { return candp + (size_t)i * CSL + sl; }, (1ull << (1 + CW)) - 1, seq,
+                                          x, a, &s_abort)) {
                         fl[h] = (int)x[0];
 #pragma unroll
                         for (int q = 0; q < CW; ++q) f[h][q] = __uint_as_float(x[1 + q]);
@@ -355,48 +394,80 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
                 }
                 fa[h] = fl[h] >= 0;
             }
-            int zero_col = -1;
-#pragma unroll 1
-            for (int c = 0; c < W; ++c) {
-                const int par = c & 1;
-                const uint32_t k0 = fa[0] ? pivot_key32(f[0][0]) : 0u, k1 = fa[1] ? pivot_key32(f[1][0]) : 0u;
-                const bool second = k1 > k0 || (k1 == k0 && k1 != 0 && fl[1] < fl[0]);
-                const uint32_t key = second ? k1 : k0;
-                const int lab = second ? fl[1] : (fa[0] ? fl[0] : INT_MAX);
-                const int wl = wave_argmax32(key, lab);
-                if (lane == wl) {  // the wave's best: key, row, slot, values
-                    float* d = s_win[par][wave];
-                    reinterpret_cast<uint32_t*>(d)[0] = key;
-                    reinterpret_cast<int*>(d)[2] = lab;
-                    reinterpret_cast<int*>(d)[3] = second ? fs[1] : fs[0];
-#pragma unroll
-                    for (int q = 0; q < CW; ++q) d[4 + q] = second ? f[1][q] : f[0][q];
-                }
-                __syncthreads();
-                int wi;
-                {
-                    const uint32_t cu = lane < CBS / 64 ? reinterpret_cast<const uint32_t*>(s_win[par][lane])[0] : 0u;
-                    const int cl = lane < CBS / 64 ? reinterpret_cast<const int*>(s_win[par][lane])[2] : INT_MAX;
-                    wi = wave_argmax32(cu, cl);
-                }
-                const float* win = s_win[par][wi];
-                const int glab = reinterpret_cast<const int*>(win)[2];
-                const float pv = win[4];
-                if (tid == 0) {
-                    s_lab[c] = glab;
-                    s_slot[c] = reinterpret_cast<const int*>(win)[3];
-                    if (pv == 0.f && zero_col < 0) zero_col = c;
-                }
-                // the pivot row's packed values: L11 (c, 0..c) from the tail slots, U11 (c, c..W)
-                if (tid < CW) s_LU[c][tid] = tid < c ? win[4 + CW - c + tid] : win[4 + tid - c];
-                float pr[CW - 1];
-                pivot_row(win + 4, pr);
+            // the steps unrolled as in the local tournament; here each candidate keeps its
+            // multipliers in place (f[h][q], q < c), so the pivot row's values are its packed row of
+            // L11 \\ U11 as they stand
+            int zero_col = -1;  // uniform
+            static_for<0, CW>([&](auto C) {
+                constexpr int c = decltype(C)::value;
+                if (c >= W) return;
+                constexpr int par = c & 1;
+                const uint32_t k0 = fa[0] ? calu_key<10>(f[0][c], (uint32_t)tid) : 0u;
+                const uint32_t k1 = fa[1] ? calu_key<10>(f[1][c], (uint32_t)(tid + CBS)) : 0u;
+                const uint32_t m = wave_max_u32(max(k0, k1));
+                if (lane == 0) s_key[par][wave] = m;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    if (fa[h] && fl[h] == glab) fa[h] = false;
-                    calu_step(f[h], pr, pv, fa[h], c);
+                    if ((h ? k1 : k0) == m && m != 0u) {
+#pragma unroll
+                        for (int q = 0; q < CW; q += 4)
+                            *reinterpret_cast<float4*>(&s_row[par][wave][q]) =
+                                make_float4(f[h][q], f[h][q + 1], f[h][q + 2], f[h][q + 3]);
+                        s_rlab[par][wave] = fl[h];
+                    }
                 }
-            }
+                __syncthreads();
+                const uint32_t M = max8_u32(s_key[par]);
+                int slot = (2 * CBS - 1) - (int)(M & (2 * CBS - 1));
+                const float* prow = s_row[par][(slot & (CBS - 1)) >> 6];
+                int glab = s_rlab[par][(slot & (CBS - 1)) >> 6];
+                if ((M >> 10) <= 1u) {
+                    // every candidate left is zero (or NaN) in this column: the one with the smallest
+                    // row index is the pivot, its multipliers stay zero (rare; two more barriers)
+                    if (zero_col < 0) zero_col = c;
+                    int lab = INT_MAX;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        if (fa[h]) lab = min(lab, fl[h]);
+                    lab = wave_min_i32(lab);
+                    if (lane == 0) s_zaux[wave] = lab;
+                    if (tid == 0) s_zaux[CBS / 64] = -1;  // stays so only when a wait gave up
+                    __syncthreads();
+                    int zl = INT_MAX;
+#pragma unroll
+                    for (int i = 0; i < CBS / 64; ++i) zl = min(zl, s_zaux[i]);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (fa[h] && fl[h] == zl) {
+#pragma unroll
+                            for (int q = 0; q < CW; ++q) s_zrow[q] = f[h][q];
+                            s_zaux[CBS / 64] = tid + h * CBS;
+                        }
+                    }
+                    __syncthreads();
+                    slot = s_zaux[CBS / 64];
+                    prow = s_zrow;
+                    glab = zl;
+                }
+                float pr[CW];
+                load_row<c>(prow, pr);
+                const float pv = pr[c];
+                const float rinv = pv != 0.f ? 1.f / pv : 0.f;
+                if (tid == 0) {
+                    s_lab[c] = glab;
+                    s_slot[c] = slot;
+                    s_dinv[c] = rinv;
+                }
+                if (tid < CW) s_LU[c][tid] = prow[tid];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (fa[h] && tid + h * CBS == slot) fa[h] = false;
+                    const float l = fa[h] ? f[h][c] * rinv : 0.f;
+                    f[h][c] = l;
+#pragma unroll
+                    for (int q = c + 1; q < CW; ++q) f[h][q] = fmaf(-l, pr[q], f[h][q]);
+                }
+            });
             __syncthreads();
             // U12 = L11^{-1} A12 of the pivot rows (from their records), one column per thread
             if (tid < rem) {
@@ -409,14 +480,14 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
                     const int sl = s_slot[c];
                     if (c < W && sl >= 0 && sl < a.P * CW) m |= 1ull << c;
                 }
-                (void)calu_poll<CW>([&](int c) { return candp + (size_t)s_slot[c] * CREC + 1 + CW + tid; }, m, seq, x, a,
+                (void)calu_poll<CW>([&](int c) { return candp + CREMW + (size_t)s_slot[c] * CREM + tid; }, m, seq, x, a,
                                     &s_abort);
 #pragma unroll
                 for (int c = 0; c < CW; ++c) u[c] = (m >> c) & 1 ? __uint_as_float(x[c]) : 0.f;
 #pragma unroll
-                for (int i = 0; i < CW - 1; ++i) {
+                for (int c = 1; c < CW; ++c) {  // row c of L11 (contiguous in LDS) against u[0..c)
 #pragma unroll
-                    for (int c = i + 1; c < CW; ++c) u[c] = fmaf(-s_LU[c][i], u[i], u[c]);
+                    for (int i = 0; i < c; ++i) u[c] = fmaf(-s_LU[c][i], u[i], u[c]);
                 }
 #pragma unroll
                 for (int c = 0; c < CW; ++c) {
@@ -449,7 +520,11 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
             for (int i = 0; i < NB; ++i) {
                 const int idx = base + i;
                 if (i >= cntw) continue;
-                if (idx < CW * CW) s_LU[idx / CW][idx % CW] = __uint_as_float(x[i]);
+                if (idx < CW * CW) {
+                    const float y = __uint_as_float(x[i]);
+                    s_LU[idx / CW][idx % CW] = y;
+                    if (idx / CW == idx % CW) s_dinv[idx / CW] = y != 0.f ? 1.f / y : 0.f;
+                }
                 else if (idx < CW * CW + CW * CREM) {
                     const int u = idx - CW * CW;
                     if (u % CREM < rem) s_U12[u / CREM][u % CREM] = __uint_as_float(x[i]);

@@ -125,10 +125,10 @@ int mixed_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * pw));
     LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * pw, hipHostMallocDefault));
     // tournament pivoting: candidate records, broadcast, arrival counter
-    LSX_HIP(ctx, hipMalloc(&lu->calu_cand, sizeof(uint64_t) * CALU_PMAX * CW * CREC));
+    LSX_HIP(ctx, hipMalloc(&lu->calu_cand, sizeof(uint64_t) * CCAND));
     LSX_HIP(ctx, hipMalloc(&lu->calu_bc, sizeof(uint64_t) * CBC));
     LSX_HIP(ctx, hipMalloc(&lu->calu_ctr, 256));
-    LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CALU_PMAX * CW * CREC, lu->stream));
+    LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CCAND, lu->stream));
     LSX_HIP(ctx, hipMemsetAsync(lu->calu_bc, 0, sizeof(uint64_t) * CBC, lu->stream));
     LSX_HIP(ctx, hipMemsetAsync(lu->calu_ctr, 0, 256, lu->stream));
     LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
@@ -240,7 +240,7 @@ int mixed_getrf(lsx_lu* lu) {
                    int64_t N, int64_t K, int) { gemm_lowp(lu, A, lda, B, ldb, C, ldc, M, N, K); };
     if (lu->calu_seq > 0xF0000000u || lu->calu_cbase > 0xF0000000u) {
         LSX_HIP(ctx, hipMemsetAsync(lu->calu_ctr, 0, 256, lu->stream));
-        LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CALU_PMAX * CW * CREC, lu->stream));
+        LSX_HIP(ctx, hipMemsetAsync(lu->calu_cand, 0, sizeof(uint64_t) * CCAND, lu->stream));
         LSX_HIP(ctx, hipMemsetAsync(lu->calu_bc, 0, sizeof(uint64_t) * CBC, lu->stream));
         lu->calu_cbase = 0;
         lu->calu_seq = 1;
