@@ -649,16 +649,21 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
             x[i] = (w < 2 * DB * NR && w / (2 * DB) < nrhs) ? ld_relaxed_u64(m + w) : 0;
         }
     };
-    if (nj > 0) issue(0);
-    for (int t = 0; t < nj; ++t) {
-        const int j = jblk(t), buf = t & 1;
-        double lv[16];
+    // the 64 x 64 tile of block j: lane rr of group g holds F[row, j DB + 16 g + q], q < 16, in
+    // the factor's own type (converted where it is used, so that no wait for the load is placed
+    // early); tiles are loaded one block ahead of their use
+    auto tload = [&](int t, T (&lv)[16]) {
+        const int j = jblk(t);
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int64_t col = (int64_t)j * DB + 16 * g + q;
-            lv[q] = (row < n && col < n) ? (double)F[col * ld + row] : 0.0;
+            lv[q] = (row < n && col < n) ? F[col * ld + row] : T(0);
         }
-        // words of block j not yet current are read again, all together, round by round
+    };
+    // block t of the solve order: its message (issued earlier) until current, into Ys, the next
+    // block's message issued, then acc += tile * x_j
+    auto step = [&](int t, const T (&lv)[16]) {
+        const int j = jblk(t), buf = t & 1;
         const uint64_t* m = msg + (size_t)j * TMW;
         uint32_t stale = 0;
 #pragma unroll
@@ -697,9 +702,20 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
 #pragma unroll
         for (int q = 0; q < 16; ++q)
 #pragma unroll
-            for (int s = 0; s < NR; ++s) acc[s] = fma(lv[q], Ys[buf][16 * g + q][s], acc[s]);
-        // Ys is double-buffered: buffer buf is written again two iterations on, after the next
-        // barrier
+            for (int s = 0; s < NR; ++s) acc[s] = fma((double)lv[q], Ys[buf][16 * g + q][s], acc[s]);
+        // Ys is double-buffered: buffer buf is written again two blocks on, after the next barrier
+    };
+    if (nj > 0) {
+        issue(0);
+        T la[16], lb[16];
+        tload(0, la);
+        for (int t = 0; t < nj; t += 2) {
+            if (t + 1 < nj) tload(t + 1, lb);
+            step(t, la);
+            if (t + 1 >= nj) break;
+            if (t + 2 < nj) tload(t + 2, la);
+            step(t + 1, lb);
+        }
     }
 #pragma unroll
     for (int s = 0; s < NR; ++s) Ps[g][rr][s] = acc[s];
@@ -712,14 +728,22 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
         if (idx < DB * NR) Yb[k][s] = bin[i] - (Ps[0][k][s] + Ps[1][k][s] + Ps[2][k][s] + Ps[3][k][s]);
     }
     __syncthreads();
+    // y = Dinv Yb: group g forms the partial products over k in [16 g, 16 g + 16) (four short
+    // chains instead of one of 64), reduced through Ps
+#pragma unroll
+    for (int s = 0; s < NR; ++s) {
+        double y = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) y = fma(Ds[rr][16 * g + q], Yb[16 * g + q][s], y);
+        Ps[g][rr][s] = y;
+    }
+    __syncthreads();
     uint64_t* mine = msg + (size_t)bw * TMW;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int s = g + 4 * q;
         if (s >= NR) continue;
-        double y = 0.0;
-#pragma unroll 8
-        for (int k = 0; k < DB; ++k) y = fma(Ds[rr][k], Yb[k][s], y);
+        const double y = (Ps[0][rr][s] + Ps[1][rr][s]) + (Ps[2][rr][s] + Ps[3][rr][s]);
         if (s < nrhs) {
             if (row < n) Out[(int64_t)s * ldout + row] = y;
             st_relaxed_u64(mine + 2 * (s * DB + rr), ll_pack((uint32_t)__double2loint(y), seq));
