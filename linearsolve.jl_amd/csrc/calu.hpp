@@ -169,8 +169,8 @@ __shared__ __align__(16) float s_U12[CW][CREM + 4];
 __shared__ float s_dinv[CW];
 
 // A live row that is not a pivot row: l U11 = a (row vector, U11 upper with its diagonal),
-// column by column (times the diagonal's reciprocals), then a_rest -= l U12.  pc0: the panel's first column from the block's first
-// row; rob: the row's byte offset in a column.  FULL: the panel has all CW columns (every panel
+// column by column (times the diagonal's reciprocals), then a_rest -= l U12.  pc0: the panel's
+// first column from the block's first row; rob: the row's byte offset in a column.  FULL: the panel has all CW columns (every panel
 // but a last partial one); only then are the multipliers stored without a test per column (a
 // test per column makes the compiler keep every address and value live).
 template <bool FULL>
@@ -178,34 +178,15 @@ __device__ __forceinline__ void calu_row_update(float* pc0, int64_t ld, uint32_t
     auto at = [&](int c) -> float& {
         return *reinterpret_cast<float*>(reinterpret_cast<char*>(pc0 + (int64_t)c * ld) + rob);
     };
-    float l[CW];
-#pragma unroll
-    for (int c = 0; c < CW; ++c) {
-        const float x = at(FULL ? c : min(c, W - 1));
-        l[c] = (FULL || c < W) ? x : 0.f;
-    }
-#pragma unroll
-    for (int i = 0; i < CW; ++i) {
-        l[i] = (FULL || i < W) ? l[i] * s_dinv[i] : 0.f;
-#pragma unroll
-        for (int c = i + 1; c < CW; ++c) l[c] = fmaf(-l[i], s_LU[i][c], l[c]);
-    }
-    if (FULL) {
-#pragma unroll
-        for (int c = 0; c < CW; ++c) at(c) = l[c];
-    } else {
-#pragma unroll 1
-        for (int c = 0; c < W; ++c) at(c) = l[c];
-    }
-    // eight columns at a time (eight loads in flight), two 16-byte LDS reads of U12 per multiplier
-    int j = 0;
-#pragma unroll 1
-    for (; j + 8 <= rem; j += 8) {
-        float x[8];
+    // the remaining columns eight at a time, each group's loads issued one group ahead of its
+    // update (the first group's with the panel values, before the multipliers are formed)
+    auto ld8 = [&](int j, float (&x)[8]) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) x[e] = at(W + j + e);
+    };
+    auto upd8 = [&](int j, float (&x)[8], const float (&l)[CW]) {
 #pragma unroll
-        for (int c = 0; c < CW; ++c) {
+        for (int c = 0; c < CW; ++c) {  // two 16-byte LDS reads of U12 per multiplier
             const float4 u0 = *reinterpret_cast<const float4*>(&s_U12[c][j]);
             const float4 u1 = *reinterpret_cast<const float4*>(&s_U12[c][j + 4]);
             x[0] = fmaf(-l[c], u0.x, x[0]);
@@ -219,9 +200,38 @@ __device__ __forceinline__ void calu_row_update(float* pc0, int64_t ld, uint32_t
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) at(W + j + e) = x[e];
+    };
+    const int ng = rem / 8;
+    float l[CW], xa[8], xb[8];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const float x = at(FULL ? c : min(c, W - 1));
+        l[c] = (FULL || c < W) ? x : 0.f;
+    }
+    if (ng > 0) ld8(0, xa);
+#pragma unroll
+    for (int i = 0; i < CW; ++i) {
+        l[i] = (FULL || i < W) ? l[i] * s_dinv[i] : 0.f;
+#pragma unroll
+        for (int c = i + 1; c < CW; ++c) l[c] = fmaf(-l[i], s_LU[i][c], l[c]);
+    }
+    if (FULL) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c) at(c) = l[c];
+    } else {
+#pragma unroll 1
+        for (int c = 0; c < W; ++c) at(c) = l[c];
     }
 #pragma unroll 1
-    for (; j < rem; ++j) {
+    for (int g = 0; g < ng; g += 2) {
+        if (g + 1 < ng) ld8(8 * (g + 1), xb);
+        upd8(8 * g, xa, l);
+        if (g + 1 >= ng) break;
+        if (g + 2 < ng) ld8(8 * (g + 2), xa);
+        upd8(8 * (g + 1), xb, l);
+    }
+#pragma unroll 1
+    for (int j = 8 * ng; j < rem; ++j) {
         float* pa = &at(W + j);
         float x = *pa;
 #pragma unroll

@@ -99,7 +99,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
                               jo + jbo, 0);
             const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
             const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
-            hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(DB), 0, lu->cur, F, ld, n, b0, DL, DU);
+            hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(256), 0, lu->cur, F, ld, n, b0, DL, DU);
             lu->prof.end(lu->cur, pi);
             if (defer)
                 hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
@@ -123,7 +123,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
         }
         const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
         const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
-        hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(DB), 0, lu->cur, F, ld, n, b0, DL, DU);
+        hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(256), 0, lu->cur, F, ld, n, b0, DL, DU);
         lu->prof.end(lu->cur, pi);
         hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
                            lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);

@@ -537,59 +537,67 @@ __global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict A, int64_t ld,
 // ------------------------------------------------------------------------------------------
 // Inverses of the 64 x 64 diagonal blocks of L (unit lower) and U (upper) of the factored
 // matrix, used by the blocked TRSM of the outer update and by the triangular solves.
-// grid = (number of blocks, 2): y = 0 inverts L blocks, y = 1 U blocks.  One thread per
-// column of the inverse (columns are independent), the block staged in LDS and read by
-// broadcast; the column being built stays in registers (loops fully unrolled) and each
-// substitution sum is split over four partial sums.
+// grid = (number of blocks, 2): y = 0 inverts L blocks, y = 1 U blocks.  Four threads per
+// column c of the inverse (columns are independent): thread p of the quad keeps the entries
+// x[4 m + p] in registers and forms the substitution sum over them; the quad's four partial sums
+// are added by two lane exchanges, so every step of the substitution is a chain of at most 16
+// products instead of 64.  The block is staged in LDS.
 template <typename T>
-__global__ __launch_bounds__(64) void k_diag_inv(const T* __restrict F, int64_t ld, int64_t n,
-                                                 int64_t b0, T* __restrict DL, T* __restrict DU) {
+__global__ __launch_bounds__(256) void k_diag_inv(const T* __restrict F, int64_t ld, int64_t n,
+                                                  int64_t b0, T* __restrict DL, T* __restrict DU) {
     const int64_t blk = b0 + blockIdx.x;
     const int64_t r0 = blk * DB;
     const int nb = (int)min((int64_t)DB, n - r0);
-    const int c = threadIdx.x;
+    const int t = threadIdx.x, c = t >> 2, p = t & 3;
     __shared__ T D[DB][DB + 1];   // D[i][k]
-#pragma unroll 16
-    for (int k = 0; k < DB; ++k) D[c][k] = (c < nb && k < nb) ? F[(r0 + k) * ld + r0 + c] : (c == k ? T(1) : T(0));
+    for (int idx = t; idx < DB * DB; idx += 256) {
+        const int i = idx & (DB - 1), k = idx / DB;
+        D[i][k] = (i < nb && k < nb) ? F[(r0 + k) * ld + r0 + i] : (i == k ? T(1) : T(0));
+    }
     __syncthreads();
-    T x[DB];
+    T xk[DB / 4];  // x[4 m + p]
+#pragma unroll
+    for (int m = 0; m < DB / 4; ++m) xk[m] = T(0);
+    auto quad_sum = [&](T v) {
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        return v;
+    };
     if (blockIdx.y == 0) {
         // column c of L^{-1}: x_c = 1, x_i = -sum_{k=c}^{i-1} L[i,k] x_k (i > c); x_k = 0 for k < c
 #pragma unroll
         for (int i = 0; i < DB; ++i) {
-            T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+            T a0 = T(0), a1 = T(0);
 #pragma unroll
-            for (int k = 0; k + 3 < i; k += 4) {
-                a0 = fma(D[i][k], x[k], a0);
-                a1 = fma(D[i][k + 1], x[k + 1], a1);
-                a2 = fma(D[i][k + 2], x[k + 2], a2);
-                a3 = fma(D[i][k + 3], x[k + 3], a3);
+            for (int m = 0; 4 * m < i; ++m) {
+                const T u = fma(D[i][4 * m + p], xk[m], (m & 1) ? a1 : a0);
+                if (m & 1) a1 = (4 * m + p < i) ? u : a1;
+                else a0 = (4 * m + p < i) ? u : a0;
             }
-#pragma unroll
-            for (int k = i & ~3; k < i; ++k) a0 = fma(D[i][k], x[k], a0);
-            x[i] = (i < c) ? T(0) : (i == c ? T(1) : -((a0 + a1) + (a2 + a3)));
+            const T sum = quad_sum(a0 + a1);
+            const T xi = (i < c) ? T(0) : (i == c ? T(1) : -sum);
+            if (p == (i & 3)) xk[i >> 2] = xi;
         }
     } else {
         // column c of U^{-1}: x_c = 1/U[c,c], x_i = -(sum_{k=i+1}^{c} U[i,k] x_k) / U[i,i] (i < c);
         // x_k = 0 for k > c
 #pragma unroll
         for (int i = DB - 1; i >= 0; --i) {
-            T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+            T a0 = T(0), a1 = T(0);
 #pragma unroll
-            for (int k = i + 1; k + 3 < DB; k += 4) {
-                a0 = fma(D[i][k], x[k], a0);
-                a1 = fma(D[i][k + 1], x[k + 1], a1);
-                a2 = fma(D[i][k + 2], x[k + 2], a2);
-                a3 = fma(D[i][k + 3], x[k + 3], a3);
+            for (int m = (i + 1) / 4; m < DB / 4; ++m) {
+                const T u = fma(D[i][4 * m + p], xk[m], (m & 1) ? a1 : a0);
+                if (m & 1) a1 = (4 * m + p > i) ? u : a1;
+                else a0 = (4 * m + p > i) ? u : a0;
             }
-#pragma unroll
-            for (int k = i + 1 + ((DB - 1 - i) & ~3); k < DB; ++k) a0 = fma(D[i][k], x[k], a0);
-            x[i] = (i > c) ? T(0) : (i == c ? T(1) / D[c][c] : -((a0 + a1) + (a2 + a3)) / D[i][i]);
+            const T sum = quad_sum(a0 + a1);
+            const T xi = (i > c) ? T(0) : (i == c ? T(1) / D[c][c] : -sum / D[i][i]);
+            if (p == (i & 3)) xk[i >> 2] = xi;
         }
     }
     T* out = (blockIdx.y == 0 ? DL : DU) + blk * DB * DB;
 #pragma unroll
-    for (int i = 0; i < DB; ++i) out[c * DB + i] = x[i];  // column-major 64 x 64
+    for (int m = 0; m < DB / 4; ++m) out[c * DB + 4 * m + p] = xk[m];  // column-major 64 x 64
 }
 
 // ------------------------------------------------------------------------------------------
