@@ -213,9 +213,11 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
                     double abstol, double reltol, int64_t maxiters, const double* pl_diag,
                     const double* pr_diag, lsx_gmres_stats* stats);
 
-/* The largest Krylov dimension a solve on A can hold: memory with restart, or maxiters without
- * (the packed triangle R and the Givens pairs live in the solving workgroups' LDS beside their
- * rows of q and the basis, so it depends on N: 128 for N <= 512, about 70 for N > 2048).
+/* The largest Krylov dimension a solve on A can hold: memory with restart, or maxiters without.
+ * 2048 for every N: the basis is in HBM, and each solving workgroup keeps the packed triangle R,
+ * the Givens pairs and z in its LDS beside its rows of q and the basis when they fit (up to 128
+ * vectors for N <= 512, about 70 for N > 2048), else in an HBM area of its own.  (The batched
+ * layout, lsx_gmres_solve_batched, keeps them in LDS and refuses dimensions beyond that.)
  * Replaces the growth of the workspace that SimpleGMRES and KrylovJL_GMRES do without restart
  * (src/simplegmres.jl:222-235 _gmres_grow_pass!/_gmres_grow_krylov!, Krylov.jl's gmres with
  * restart = false under src/iterative_wrappers.jl:693): a caller asks first and restarts at

@@ -17,6 +17,7 @@ import json
 import enum
 import math
 import os
+import warnings
 
 import numpy as np
 
@@ -714,26 +715,31 @@ class MI355XSimpleGMRES:
                 cache.abstol, cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
                 pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve_batched")
         else:
-            memory, restart = _device_krylov_dims(cache, self.memory, bool(self.restart))
+            memory, restart, kst = _device_krylov_dims(cache, self.memory, bool(self.restart))
             cache.ctx.check(_abi.lib().lsx_gmres_solve(
                 cache.cacheval.h, pb, _ptr_out(cache.u), memory, int(restart), cache.abstol,
                 cache.reltol, cache.maxiters, pl.ctypes.data if pl is not None else None,
                 pr.ctypes.data if pr is not None else None, ctypes.byref(st)), "lsx_gmres_solve")
+            return LinearSolution(cache.u, st.resid, ReturnCode(st.retcode), int(st.iters), cache, kst)
         return LinearSolution(cache.u, st.resid, ReturnCode(st.retcode), int(st.iters), cache)
 
 
 def _device_krylov_dims(cache, memory, restart):
-    """(memory, restart) for the device solve.  Without restart the reference grows the Krylov
-    basis up to maxiters vectors (src/simplegmres.jl:222-235; Krylov.jl's gmres likewise); the
-    device holds at most lsx_gmres_max_memory of them, so a solve that could need more restarts
-    every that many iterations instead."""
+    """(memory, restart, stats) for the device solve.  Without restart the reference grows the
+    Krylov basis up to maxiters vectors (src/simplegmres.jl:222-235; Krylov.jl's gmres likewise);
+    the device holds at most lsx_gmres_max_memory of them (2048: the basis in HBM, its
+    Hessenberg/Givens bookkeeping in LDS or, beyond what the LDS holds, in HBM).  A solve whose
+    maxiters exceeds that restarts every that many iterations instead: it says so with a
+    RuntimeWarning and in the solution's stats ({"krylov_restart": that length})."""
     if restart:
-        return memory, True
+        return memory, True, None
     lim = ctypes.c_int32()
     cache.ctx.check(_abi.lib().lsx_gmres_max_memory(cache.cacheval.h, ctypes.byref(lim)), "lsx_gmres_max_memory")
     if max(memory, cache.maxiters) <= lim.value:
-        return memory, False
-    return int(lim.value), True
+        return memory, False, None
+    warnings.warn(f"GMRES without restart: the device holds at most {lim.value} Krylov vectors, so this solve "
+                  f"(maxiters = {cache.maxiters}) restarts every {lim.value} iterations", RuntimeWarning, stacklevel=3)
+    return int(lim.value), True, {"krylov_restart": int(lim.value)}
 
 
 class MI355XKrylovJL_GMRES:
@@ -797,7 +803,7 @@ class MI355XKrylovJL_GMRES:
             memory = int(self.memory)
         else:
             memory = self.gmres_restart if self.gmres_restart > 0 else min(20, n)
-        memory, restart = _device_krylov_dims(cache, memory, self.gmres_restart > 0)
+        memory, restart, kst = _device_krylov_dims(cache, memory, self.gmres_restart > 0)
         st = _abi.GmresStats()
         cache.ctx.check(_abi.lib().lsx_gmres_solve(
             cache.cacheval.h, pb, _ptr_out(cache.u), memory, int(restart), cache.abstol, cache.reltol,
@@ -806,7 +812,7 @@ class MI355XKrylovJL_GMRES:
         rc = ReturnCode(st.retcode)
         if rc == ReturnCode.Infeasible:
             rc = ReturnCode.Failure
-        return LinearSolution(cache.u, st.resid, rc, int(st.iters), cache)
+        return LinearSolution(cache.u, st.resid, rc, int(st.iters), cache, kst)
 
 
 # -------------------------------------------------------------------------------------------------

@@ -223,13 +223,15 @@ __device__ void sym_givens(double a, double b, double& c, double& s, double& rho
 template <int RPT>
 __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& sh, double* dyn) {
     const int cap = a.cap;
-    double* R = dyn;                          // cap (cap + 1) / 2
-    double* cs = R + (cap * (cap + 1)) / 2;   // cap
+    // the bookkeeping, read and written by thread 0 (z also by every thread after a barrier): in
+    // LDS, or in this workgroup's HBM area when the Krylov dimension is too large for the LDS
+    double* R = a.Rg ? a.Rg + (size_t)w * (size_t)gmres_book_words(cap) : dyn;  // cap (cap + 1) / 2
+    double* cs = R + ((size_t)cap * (cap + 1)) / 2;  // cap
     double* sn = cs + cap;                    // cap
     double* z = sn + cap;                     // cap + 1
     // q and the current basis slice, one LDS column per thread (each thread touches only its own
     // slots, so no synchronization is needed for them)
-    double* qs = z + cap + 1;                 // RPT x GBS
+    double* qs = a.Rg ? dyn : z + cap + 1;    // RPT x GBS
     double* vs = qs + RPT * GBS;              // RPT x GBS
     __shared__ int s_flags[4];
     GmresCtx<RPT> g(a, sh, w);
@@ -525,37 +527,41 @@ int csr_spmv_dev(lsx_csr* A, const double* dx, double* dy, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? LSX_OK : LSX_EHIP;
 }
 
-// Dynamic LDS of a solve: the packed triangle R, the Givens pairs and z, q and the basis slice.
-static size_t gmres_dynlds(int64_t cap, int rpt) {
-    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 3 * (size_t)cap + 1 + 2 * (size_t)rpt * GBS);
+// Dynamic LDS of a solve: q and the basis slice, and (rglob false) the packed triangle R, the
+// Givens pairs and z.
+static size_t gmres_dynlds(int64_t cap, int rpt, bool rglob) {
+    return sizeof(double) * ((rglob ? 0 : (size_t)gmres_book_words(cap)) + 2 * (size_t)rpt * GBS);
 }
 constexpr size_t GMRES_LDS_MAX = 150 * 1024;
 
-// The largest Krylov dimension whose solve state fits a workgroup, for a matrix of N rows.
-int gmres_cap_limit(int64_t N) {
+// The largest Krylov dimension of one solve: GMRES_MAX_CAP_G (bookkeeping in HBM from the LDS
+// bound on); with hbm false (the multi-solve launches) the largest whose bookkeeping fits the LDS
+// of a workgroup of a matrix of N rows.
+int gmres_cap_limit(int64_t N, bool hbm) {
+    if (hbm) return GMRES_MAX_CAP_G;
     int P = 1;
     const int rpt = pick_rpt(N, P);
     int cap = GMRES_MAX_CAP;
-    while (cap > 1 && gmres_dynlds(cap, rpt) > GMRES_LDS_MAX) --cap;
+    while (cap > 1 && gmres_dynlds(cap, rpt, false) > GMRES_LDS_MAX) --cap;
     return cap;
 }
 
-int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out) {
+int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out, bool hbm) {
     int P = 1;
     const int rpt = pick_rpt(N, P);
     if (P > PMAX_G) return fail(ctx, LSX_EINVAL, "gmres: N = %lld exceeds %d rows", (long long)N, PMAX_G * 16 * GBS);
     const int64_t memory = std::max<int64_t>(1, std::min<int64_t>(p.memory, p.maxiters));
     const int64_t cap = p.restart ? memory : std::max<int64_t>(memory, p.maxiters);
-    if (cap > GMRES_MAX_CAP)
-        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld exceeds %d (restart = %d)", (long long)cap,
-                    GMRES_MAX_CAP, p.restart);
-    const size_t dynlds = gmres_dynlds(cap, rpt);
-    if (dynlds > GMRES_LDS_MAX)
-        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld too large for N = %lld", (long long)cap, (long long)N);
+    const int lim = gmres_cap_limit(N, hbm);
+    if (cap > lim)
+        return fail(ctx, LSX_EINVAL, "gmres: Krylov dimension %lld exceeds %d (restart = %d, N = %lld)", (long long)cap,
+                    lim, p.restart, (long long)N);
+    const bool rglob = gmres_dynlds(cap, rpt, false) > GMRES_LDS_MAX;
     out->P = P;
     out->rpt = rpt;
     out->cap = (int)cap;
-    out->dynlds = dynlds;
+    out->rglob = rglob;
+    out->dynlds = gmres_dynlds(cap, rpt, rglob);
     return LSX_OK;
 }
 
@@ -577,7 +583,7 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     lsx_ctx* ctx = A->ctx;
     const int64_t N = A->N;
     GmresPlan plan;
-    int rc = gmres_plan(ctx, N, p, &plan);
+    int rc = gmres_plan(ctx, N, p, &plan, true);
     if (rc) return rc;
     const int P = plan.P, rpt = plan.rpt;
     const int64_t memory = std::max<int64_t>(1, std::min<int64_t>(p.memory, p.maxiters));
@@ -591,6 +597,17 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
         if (hipMalloc(&A->ws, need) != hipSuccess)
             return fail(ctx, LSX_ENOMEM, "gmres: cannot allocate %zu bytes of Krylov workspace", need);
         A->ws_bytes = need;
+    }
+    if (plan.rglob) {
+        const size_t rneed = sizeof(double) * (size_t)P * (size_t)gmres_book_words(cap);
+        if (rneed > A->rg_bytes) {
+            (void)hipFree(A->rg);
+            A->rg = nullptr;
+            A->rg_bytes = 0;
+            if (hipMalloc(&A->rg, rneed) != hipSuccess)
+                return fail(ctx, LSX_ENOMEM, "gmres: cannot allocate %zu bytes of Krylov bookkeeping", rneed);
+            A->rg_bytes = rneed;
+        }
     }
     LSX_HIP(ctx, hipMemsetAsync(A->red, 0, sizeof(uint64_t) * 4 * PMAX_G + 256, st));
     LSX_HIP(ctx, hipMemsetAsync(A->out, 0, sizeof(GmresOut), st));
@@ -608,6 +625,7 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     g.memory = (int)memory;
     g.restart = p.restart;
     g.cap = (int)cap;
+    g.Rg = plan.rglob ? A->rg : nullptr;
     g.maxiters = p.maxiters;
     g.abstol = p.abstol;
     g.reltol = p.reltol;
@@ -684,6 +702,7 @@ static void csr_free(lsx_csr* A) {
     (void)hipFree(A->col);
     (void)hipFree(A->val);
     (void)hipFree(A->ws);
+    (void)hipFree(A->rg);
     (void)hipFree(A->red);
     (void)hipFree(A->out);
     if (A->hout) (void)hipHostFree(A->hout);
@@ -831,7 +850,7 @@ int lsx_gmres_solve(lsx_csr* A, const double* b, double* x, int32_t memory, int3
 
 int lsx_gmres_max_memory(lsx_csr* A, int32_t* memory) {
     if (!A || !memory) return LSX_EINVAL;
-    *memory = gmres_cap_limit(A->N);
+    *memory = gmres_cap_limit(A->N, true);
     return LSX_OK;
 }
 

@@ -4,7 +4,8 @@
 
 namespace lsx {
 constexpr int PMAX_G = 256;         // max workgroups of one GMRES solve (8192 rows each)
-constexpr int GMRES_MAX_CAP = 128;  // max Krylov dimension held in LDS
+constexpr int GMRES_MAX_CAP = 128;     // max Krylov dimension whose bookkeeping is held in LDS
+constexpr int GMRES_MAX_CAP_G = 2048;  // max Krylov dimension of one solve (bookkeeping in HBM)
 
 struct GmresOut {
     int32_t retcode;
@@ -35,7 +36,9 @@ struct GmresArgs {
     double* xr;         // N: correction of the current pass (restart)
     int memory;
     int restart;
-    int cap;            // size of the LDS bookkeeping (memory, or maxiters when not restarted)
+    int cap;            // size of the bookkeeping (memory, or maxiters when not restarted)
+    double* Rg = nullptr;  // non-null: each workgroup's bookkeeping (R, Givens pairs, z) in HBM at
+                        // Rg + w gmres_book_words(cap) instead of LDS
     int64_t maxiters;
     double abstol, reltol;
     int P;
@@ -50,9 +53,13 @@ struct GmresArgs {
 struct GmresPlan {
     int P;          // workgroups
     int rpt;        // rows per thread (1, 4 or 16)
-    int cap;        // Krylov dimension held in LDS
+    int cap;        // Krylov dimension
+    bool rglob;     // the bookkeeping is in HBM (it does not fit the LDS beside q and the basis slice)
     size_t dynlds;  // dynamic LDS bytes per workgroup
 };
+
+// Doubles of one workgroup's bookkeeping: the packed triangle R, the Givens pairs and z.
+__host__ __device__ inline int64_t gmres_book_words(int64_t cap) { return cap * (cap + 1) / 2 + 3 * cap + 1; }
 
 // 64-bit words of a solve's exchange area (all-reduce slots, barrier counter, published totals);
 // zero before use.
@@ -68,6 +75,8 @@ struct lsx_csr {
     hipStream_t stream = nullptr;
     double* ws = nullptr;        // Krylov basis + correction
     size_t ws_bytes = 0;
+    double* rg = nullptr;        // per-workgroup bookkeeping in HBM (Krylov dimensions beyond the LDS)
+    size_t rg_bytes = 0;
     uint64_t* red = nullptr;     // all-reduce slots + barrier counter
     lsx::GmresOut* out = nullptr;
     lsx::GmresOut* hout = nullptr;  // pinned
@@ -94,7 +103,9 @@ namespace lsx {
 int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx, const double* dpl,
                  const double* dpr, hipStream_t st);
 // Workgroups, rows per thread and LDS of a solve; LSX_EINVAL (message in ctx) when it does not fit.
-int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out);
+// hbm: the Krylov dimension may exceed what the LDS holds (single solves, gmres_launch); the
+// multi-solve launches keep the bookkeeping in LDS.
+int gmres_plan(lsx_ctx* ctx, int64_t N, const GmresParams& p, GmresPlan* out, bool hbm = false);
 // Solves of one rows-per-thread class in one launch on `pool` workgroups (at least the widest
 // solve's count): rank r of solve i is work number first[i] + r (first has nitems + 1 entries,
 // first[nitems] = total); workgroups take numbers from *ticket as they start and after each solve,

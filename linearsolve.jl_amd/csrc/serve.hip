@@ -578,6 +578,7 @@ void run_gmres_group(lsx_serve* s, GmresLane& L, std::vector<Job*>& group) {
         a.memory = (int)std::max<int64_t>(1, std::min<int64_t>(r.memory, r.maxiters));
         a.restart = r.restart ? 1 : 0;
         a.cap = j->plan.cap;
+        a.Rg = nullptr;  // bookkeeping in LDS (the plan is made without the HBM option)
         a.maxiters = r.maxiters;
         a.abstol = r.abstol;
         a.reltol = r.reltol;

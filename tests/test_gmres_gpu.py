@@ -347,8 +347,8 @@ def test_krylovjl_gmres_resize(lsx):
 def test_krylovjl_gmres_options_map_to_the_device_solve(lsx):
     # gmres_restart = 0: no restart, memory = min(20, n) (src/iterative_wrappers.jl:211-214): the
     # same solve as SimpleGMRES(restart = false, memory = 20), and the oracle's; gmres_restart = k:
-    # SimpleGMRES(restart = true, memory = k).  Without restart and maxiters above the basis the
-    # device holds (128 vectors at N = 400), the solve restarts every 128 iterations.
+    # SimpleGMRES(restart = true, memory = k).  Without restart the basis grows to maxiters
+    # (here 400 vectors: beyond the 128 whose bookkeeping the LDS holds at N = 400).
     P = problems(lsx)
     rp, ci, v = P.convection_diffusion(20, 2.0)
     N = 400
@@ -369,33 +369,82 @@ def test_krylovjl_gmres_options_map_to_the_device_solve(lsx):
     assert k5.iters == s5.iters
     big = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XKrylovJL_GMRES(), maxiters=400, Pl=d, abstol=0.0,
                     reltol=1e-13)
-    ref = O.gmres(rp, ci, v, b, memory=128, restart=True, abstol=0.0, reltol=1e-13, maxiters=400, pl=d)
-    assert big.iters == ref["iters"] and int(big.retcode) == ref["retcode"]
+    ref = O.gmres(rp, ci, v, b, memory=20, restart=False, abstol=0.0, reltol=1e-13, maxiters=400, pl=d)
+    assert big.iters == ref["iters"] and int(big.retcode) == ref["retcode"] and big.stats is None
     np.testing.assert_allclose(big.u, ref["u"], rtol=1e-8, atol=1e-12)
 
 
 def test_max_memory_bounds_the_unrestarted_basis(lsx):
-    # lsx_gmres_max_memory: 128 vectors for small N, fewer as each workgroup's rows grow; a solve
-    # without restart at that bound runs, one vector more is refused with LSX_EINVAL
+    # lsx_gmres_max_memory: 2048 vectors whatever N (the bookkeeping moves from LDS to HBM where
+    # the LDS cannot hold it); a solve without restart at that bound runs, one vector more is
+    # refused with LSX_EINVAL, and the batched layout (bookkeeping in LDS) refuses dimensions the
+    # LDS cannot hold
     import ctypes
     from linearsolve_jl_amd import _abi
     P = problems(lsx)
     L = _abi.lib()
-    prev = 129
-    for m in (8, 22, 45, 90, 200):
+    for m in (8, 45, 90):
         rp, ci, v = P.convection_diffusion(m, 1.0)
         N = m * m
         h = lsx._CSRHandle(lsx.Context.get(0), lsx.CSRMatrix(rp, ci, v, (N, N)))
         lim = ctypes.c_int32()
         assert L.lsx_gmres_max_memory(h.h, ctypes.byref(lim)) == 0
-        assert 1 <= lim.value <= prev and (N > 512 or lim.value == 128), (m, lim.value)
-        prev = lim.value
+        assert lim.value == 2048, (m, lim.value)
         b = np.ones(N)
         x = np.empty(N)
         st = _abi.GmresStats()
-        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-12, lim.value,
+        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-8, lim.value,
                                None, None, ctypes.byref(st))
-        assert rc == 0 and st.iters <= lim.value
-        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-12, lim.value + 1,
+        assert rc == 0 and st.retcode == 1 and st.iters <= lim.value
+        S = sp.csr_matrix((v, ci, rp), shape=(N, N))
+        assert np.linalg.norm(b - S @ x) <= 1.01e-8 * np.linalg.norm(b)
+        rc = L.lsx_gmres_solve(h.h, b.ctypes.data, x.ctypes.data, lim.value, 0, 0.0, 1e-8, lim.value + 1,
                                None, None, ctypes.byref(st))
         assert rc == _abi.LSX_EINVAL
+    rp, ci, v = P.convection_diffusion(64, 1.0)
+    N = 64 * 64
+    h = lsx._CSRHandle(lsx.Context.get(0), lsx.CSRMatrix(rp, ci, v, (N, N)))
+    b = np.ones(N)
+    x = np.empty(N)
+    st = _abi.GmresStats()
+    rc = L.lsx_gmres_solve_batched(h.h, N, b.ctypes.data, x.ctypes.data, 1000, 0, 0.0, 1e-8, 1000, None, None,
+                                   ctypes.byref(st))
+    assert rc == _abi.LSX_EINVAL
+
+
+def test_unrestarted_beyond_lds_matches_oracle(lsx):
+    # N = 4096: the LDS beside each workgroup's rows holds the bookkeeping of about 70 vectors; this
+    # solve needs 218 without restart, so R, the Givens pairs and z live in HBM.  Iterates,
+    # iteration count and retcode are the oracle's SimpleGMRES(restart = false)
+    # (src/simplegmres.jl:222-235, 500-606).
+    P = problems(lsx)
+    m = 64
+    rp, ci, v = P.convection_diffusion(m, 2.0)
+    N = m * m
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    b = np.random.default_rng(3).standard_normal(N)
+    d = np.full(N, 4.0)
+    sol = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XSimpleGMRES(restart=False, memory=20), Pl=d,
+                    abstol=0.0, reltol=1e-10, maxiters=600)
+    ref = O.gmres(rp, ci, v, b, memory=20, restart=False, abstol=0.0, reltol=1e-10, maxiters=600, pl=d)
+    print(f"unrestarted N = {N}: {sol.iters} iterations (oracle {ref['iters']})", flush=True)
+    assert ref["iters"] > 100
+    assert sol.iters == ref["iters"] and int(sol.retcode) == ref["retcode"] and sol.stats is None
+    np.testing.assert_allclose(sol.u, ref["u"], rtol=1e-8, atol=1e-10 * np.abs(ref["u"]).max())
+
+
+def test_unrestarted_beyond_the_device_bound_warns(lsx):
+    # maxiters above 2048 without restart: the solve restarts every 2048 iterations and says so
+    P = problems(lsx)
+    rp, ci, v = P.convection_diffusion(20, 2.0)
+    N = 400
+    A = lsx.CSRMatrix(rp, ci, v, (N, N))
+    b = np.random.default_rng(4).standard_normal(N)
+    with pytest.warns(RuntimeWarning, match="restarts every 2048"):
+        sol = lsx.solve(lsx.LinearProblem(A, b.copy()), lsx.MI355XSimpleGMRES(restart=False), maxiters=3000,
+                        abstol=0.0, reltol=1e-10)
+    assert sol.stats == {"krylov_restart": 2048}
+    assert sol.retcode == lsx.ReturnCode.Success
+    ref = O.gmres(rp, ci, v, b, memory=2048, restart=True, abstol=0.0, reltol=1e-10, maxiters=3000)
+    assert sol.iters == ref["iters"]
+    np.testing.assert_allclose(sol.u, ref["u"], rtol=1e-8, atol=1e-12)
