@@ -219,19 +219,21 @@ __device__ void sym_givens(double a, double b, double& c, double& s, double& rho
     }
 }
 
-// The solve on workgroup w of a.P (sh, dyn: the workgroup's static and dynamic LDS).
-template <int RPT>
+// The solve on workgroup w of a.P (sh, dyn: the workgroup's static and dynamic LDS).  RG: the
+// bookkeeping is in HBM (a.Rg) instead of LDS; a compile-time choice, so that each pointer below
+// is known to be LDS or global memory and gets that memory's instructions.
+template <int RPT, bool RG>
 __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& sh, double* dyn) {
     const int cap = a.cap;
     // the bookkeeping, read and written by thread 0 (z also by every thread after a barrier): in
     // LDS, or in this workgroup's HBM area when the Krylov dimension is too large for the LDS
-    double* R = a.Rg ? a.Rg + (size_t)w * (size_t)gmres_book_words(cap) : dyn;  // cap (cap + 1) / 2
+    double* R = RG ? a.Rg + (size_t)w * (size_t)gmres_book_words(cap) : dyn;  // cap (cap + 1) / 2
     double* cs = R + ((size_t)cap * (cap + 1)) / 2;  // cap
     double* sn = cs + cap;                    // cap
     double* z = sn + cap;                     // cap + 1
     // q and the current basis slice, one LDS column per thread (each thread touches only its own
     // slots, so no synchronization is needed for them)
-    double* qs = a.Rg ? dyn : z + cap + 1;    // RPT x GBS
+    double* qs = dyn + (RG ? 0 : gmres_book_words(cap));  // RPT x GBS
     double* vs = qs + RPT * GBS;              // RPT x GBS
     __shared__ int s_flags[4];
     GmresCtx<RPT> g(a, sh, w);
@@ -448,11 +450,11 @@ __device__ __forceinline__ void gmres_body(const GmresArgs& a, int w, GShared& s
     }
 }
 
-template <int RPT>
+template <int RPT, bool RG>
 __global__ __launch_bounds__(GBS) void k_gmres(GmresArgs a) {
     __shared__ GShared sh;
     extern __shared__ double dyn[];
-    gmres_body<RPT>(a, (int)blockIdx.x, sh, dyn);
+    gmres_body<RPT, RG>(a, (int)blockIdx.x, sh, dyn);
 }
 
 // Several solves in one launch (gmres_launch_multi), on a pool of workgroups that may be smaller
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(GBS) void k_gmres_multi(const GmresArgs* __restrict
         }
         const int item = __builtin_amdgcn_readfirstlane(lo);
         const GmresArgs a = items[item];
-        gmres_body<RPT>(a, id - first[item], sh, dyn);
+        gmres_body<RPT, false>(a, id - first[item], sh, dyn);
     }
 }
 
@@ -636,9 +638,18 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
     g.timeout = wait_ticks(ctx, 20.0);
     const size_t dynlds = plan.dynlds;
     switch (rpt) {
-        case 1: hipLaunchKernelGGL(k_gmres<1>, dim3(P), dim3(GBS), dynlds, st, g); break;
-        case 4: hipLaunchKernelGGL(k_gmres<4>, dim3(P), dim3(GBS), dynlds, st, g); break;
-        default: hipLaunchKernelGGL(k_gmres<16>, dim3(P), dim3(GBS), dynlds, st, g); break;
+        case 1:
+            if (plan.rglob) hipLaunchKernelGGL((k_gmres<1, true>), dim3(P), dim3(GBS), dynlds, st, g);
+            else hipLaunchKernelGGL((k_gmres<1, false>), dim3(P), dim3(GBS), dynlds, st, g);
+            break;
+        case 4:
+            if (plan.rglob) hipLaunchKernelGGL((k_gmres<4, true>), dim3(P), dim3(GBS), dynlds, st, g);
+            else hipLaunchKernelGGL((k_gmres<4, false>), dim3(P), dim3(GBS), dynlds, st, g);
+            break;
+        default:
+            if (plan.rglob) hipLaunchKernelGGL((k_gmres<16, true>), dim3(P), dim3(GBS), dynlds, st, g);
+            else hipLaunchKernelGGL((k_gmres<16, false>), dim3(P), dim3(GBS), dynlds, st, g);
+            break;
     }
     LSX_HIP(ctx, hipGetLastError());
     return LSX_OK;
