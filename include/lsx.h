@@ -80,7 +80,10 @@ int lsx_synchronize(lsx_ctx* ctx);
 /* Handle of one n x n factorization kept resident on the device: replaces the LU object that
  * LUFactorization stores in cache.cacheval (src/factorization.jl:714, init_cacheval :840-849).
  * dtype LSX_F64 = plain partial-pivot LU (LAPACK dgetrf semantics); LSX_F32 / LSX_BF16 = the
- * mixed-precision factorization used by lsx_lu_refine. */
+ * mixed-precision factorization used by lsx_lu_refine: FP32 factors with tournament pivoting
+ * on block columns of at most 16384 rows (its pivots need not be LAPACK's; taller ones take exact
+ * partial pivoting), their trailing products on BF16 MFMA from three (LSX_F32: FP32 accuracy)
+ * or two (LSX_BF16: relative error ~2^-17) BF16 pieces per operand. */
 int lsx_lu_create(lsx_ctx* ctx, int64_t n, int dtype, lsx_lu** out);
 int lsx_lu_destroy(lsx_lu* lu);
 

@@ -429,9 +429,11 @@ class MI355XLUFactorization(_DenseAlg):
 
 
 class MI355XMixedLUFactorization(_DenseAlg):
-    """Mixed-precision LU: factor in FP32 or BF16 (FP32 accumulate) on the GPU, recover FP64
-    accuracy by iterative refinement or GMRES-IR with FP64 residuals.  API shape of
-    OpenBLAS32MixedLUFactorization (src/openblas.jl:461-543), which stops at FP32 accuracy."""
+    """Mixed-precision LU: factor with FP32 arithmetic ("fp32": trailing products of three BF16
+    pieces per operand, FP32 accuracy) or BF16 products ("bf16": two pieces, relative error about
+    2^-17), FP32 accumulation, on the GPU; recover FP64 accuracy by iterative refinement or
+    GMRES-IR with FP64 residuals.  API shape of OpenBLAS32MixedLUFactorization
+    (src/openblas.jl:461-543), which stops at FP32 accuracy."""
 
     def __init__(self, precision="fp32", refinement="auto", tol=1e-13, max_iters=50):
         if precision not in ("fp32", "bf16"):

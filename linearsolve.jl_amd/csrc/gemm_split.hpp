@@ -1,5 +1,6 @@
-// gemm_split.hpp — the FP32 trailing update of the mixed-precision LU on BF16 MFMA
-// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), with each FP32 operand split into S BF16 pieces:
+// gemm_split.hpp — the trailing update of the mixed-precision LU on BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), with each FP32 operand split into S BF16 pieces
+// (S = 3 for FP32 factors, S = 2 for BF16 factors):
 //   x = x0 + x1 + x2 (+ r),  x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1),
 // every piece within 2^-9 of what is left, so |r| <= 2^-9S |x|.  A product x y is formed from the
 // pieces' products whose order (i + j) is below S:
@@ -16,8 +17,6 @@
 // at a time, split into BF16 pieces as it is stored in LDS
 // (A as [row][k], B as [col][k], so a lane's MFMA operand is one 16-byte read).  The MFMA puts
 // four consecutive rows of one C column in a lane, so C is read and written as float4.
-// SCALE (S = 1 only): column j of B is multiplied by the power of two scale[j] before rounding and
-// the product's column by 1 / scale[j] after (both exact).
 #pragma once
 #include "common.hpp"
 
@@ -45,11 +44,10 @@ __device__ __forceinline__ void split8(const float (&x)[8], sv8bf (&p)[S]) {
     }
 }
 
-template <int S, bool SCALE>
+template <int S>
 __global__ __launch_bounds__(512, 1) void k_gemm_split(const float* __restrict Ag, int64_t lda,
                                                       const float* __restrict Bg, int64_t ldb, float* Cg,
-                                                      int64_t ldc, int M, int N, int K,
-                                                      const float* __restrict scale) {
+                                                      int64_t ldc, int M, int N, int K) {
     constexpr int GGM_ = 8;
     const int TM = (M + SBM - 1) / SBM, TN = (N + SBN - 1) / SBN, NTL = TM * TN;
     // XCD-aware tile order: workgroup b runs on XCD b mod 8, which takes a contiguous range of the
@@ -87,21 +85,20 @@ __global__ __launch_bounds__(512, 1) void k_gemm_split(const float* __restrict A
             ra[e] = (gr < M && gk < K) ? Ag[(int64_t)gk * lda + gr] : 0.f;
         }
         const int gc = col0 + bc;
-        const float sc = (SCALE && gc < N) ? scale[gc] : 1.f;
         const float* bp = Bg + (int64_t)gc * ldb + k0 + bkb * 8;
         if (bvec && gc < N && k0 + bkb * 8 + 7 < K) {
             const sv4f u0 = *reinterpret_cast<const sv4f*>(bp);
             const sv4f u1 = *reinterpret_cast<const sv4f*>(bp + 4);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                rb[e] = u0[e] * sc;
-                rb[e + 4] = u1[e] * sc;
+                rb[e] = u0[e];
+                rb[e + 4] = u1[e];
             }
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int gk = k0 + bkb * 8 + e;
-                rb[e] = (gc < N && gk < K) ? bp[e] * sc : 0.f;
+                rb[e] = (gc < N && gk < K) ? bp[e] : 0.f;
             }
         }
     };
@@ -174,14 +171,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_split(const float* __restrict A
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int col = col0 + wn * 32 + j * 16 + (lane & 15);
-        const float inv = (SCALE && col < N) ? 1.f / scale[col] : 1.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = row0 + wm * 64 + i * 16 + 4 * (lane >> 4);
             float* cp = Cg + (int64_t)col * ldc + row;
             sv4f nv;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) nv[q] = SCALE ? cv[i][j][q] - acc[i][j][q] * inv : cv[i][j][q] - acc[i][j][q];
+            for (int q = 0; q < 4; ++q) nv[q] = cv[i][j][q] - acc[i][j][q];
             if (col < N && vec && row + 3 < M) {
                 *reinterpret_cast<sv4f*>(cp) = nv;
             } else {
@@ -193,19 +189,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_split(const float* __restrict A
     }
 }
 
-// C -= A B with S BF16 pieces per operand on stream st (scale: per-column powers of two of B,
-// S = 1 only, or null).
+// C -= A B with S BF16 pieces per operand on stream st.
 template <int S>
 static void launch_gemm_split(hipStream_t st, const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
-                              int64_t ldc, int64_t M, int64_t N, int64_t K, const float* scale) {
+                              int64_t ldc, int64_t M, int64_t N, int64_t K) {
     const int64_t T_ = cdiv(M, SBM) * cdiv(N, SBN);
     const dim3 grid((unsigned)(8 * cdiv(T_, 8)));
-    if (S == 1 && scale)
-        hipLaunchKernelGGL((k_gemm_split<1, true>), grid, dim3(512), 0, st, A, lda, B, ldb, C, ldc, (int)M, (int)N,
-                           (int)K, scale);
-    else
-        hipLaunchKernelGGL((k_gemm_split<S, false>), grid, dim3(512), 0, st, A, lda, B, ldb, C, ldc, (int)M, (int)N,
-                           (int)K, scale);
+    hipLaunchKernelGGL((k_gemm_split<S>), grid, dim3(512), 0, st, A, lda, B, ldb, C, ldc, (int)M, (int)N, (int)K);
 }
 
 }  // namespace lsx

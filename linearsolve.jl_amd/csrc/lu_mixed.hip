@@ -1,6 +1,7 @@
-// lu_mixed.hip — mixed-precision LU: factor in FP32 (FP32 MFMA trailing update) or with BF16
-// trailing updates (BF16 operands with per-column power-of-two scales, FP32 accumulation), then
-// recover FP64 accuracy by iterative refinement (IR) or GMRES-IR with FP64 residuals.
+// lu_mixed.hip — mixed-precision LU: factor with FP32 arithmetic (trailing products of three BF16
+// pieces per operand, FP32 accumulation: FP32 accuracy at BF16 MFMA rates) or with BF16 trailing
+// products (two pieces per operand, relative error ~2^-17), with tournament pivoting, then recover
+// FP64 accuracy by iterative refinement (IR) or GMRES-IR with FP64 residuals.
 //
 // The API shape follows OpenBLAS32MixedLUFactorization (src/openblas.jl:461-543: cast A to
 // FP32, sgetrf, sgetrs, cast back), which stops at FP32 accuracy; here the low-precision factors
@@ -27,8 +28,6 @@ struct MixedState {
     float* DL = nullptr;     // FP32 inverses of diagonal blocks
     float* DU = nullptr;
     double* anorm = nullptr; // device: ||A||_inf
-    float* scale = nullptr;  // device: per-column power-of-two scales of the U12 operand (BF16 path),
-                             // n floats per getrf stream
     double* r = nullptr;     // n x cap residuals / corrections
     double* xw = nullptr;    // n x cap iterate
     double* bw = nullptr;    // n x cap right-hand sides
@@ -50,20 +49,6 @@ __global__ void k_cast_f32(const double* __restrict src, int64_t lds, float* __r
     const int64_t col = blockIdx.y;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[col * ldd + i] = (float)src[col * lds + i];
-}
-
-// Per-column power-of-two scales of the U12 block of an update (K x N, column-major, ld):
-// out[j] = 2^-e with 2^e >= max_k |U[k, j]|, so every scaled column has magnitude <= 1 before it
-// is rounded to BF16 (and the scaling is exact).  One thread per column.
-__global__ void k_colmax_scale(const float* __restrict U, int64_t ld, int K, int N, float* __restrict out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
-    const float* col = U + (int64_t)j * ld;
-    float m = 0.f;
-    for (int k = 0; k < K; ++k) m = fmaxf(m, fabsf(col[k]));
-    int e = 0;
-    if (m > 0.f && isfinite(m)) (void)frexpf(m, &e);
-    out[j] = ldexpf(1.f, -e);
 }
 
 // Y += alpha X, column-major n x nrhs.
@@ -120,7 +105,6 @@ int mixed_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipMalloc(&m->DL, sizeof(float) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&m->DU, sizeof(float) * DB * DB * nblk));
     LSX_HIP(ctx, hipMalloc(&m->anorm, sizeof(double)));
-    LSX_HIP(ctx, hipMalloc(&m->scale, sizeof(float) * 2 * (lu->n + 64)));
     const int64_t pw = std::max<int64_t>(resid_part_words(n), cdiv(n, 256) * 64);
     LSX_HIP(ctx, hipMalloc(&m->part, sizeof(double) * pw));
     LSX_HIP(ctx, hipHostMalloc(&m->hpart, sizeof(double) * pw, hipHostMallocDefault));
@@ -146,7 +130,6 @@ void mixed_free(lsx_lu* lu) {
     (void)hipFree(m->DL);
     (void)hipFree(m->DU);
     (void)hipFree(m->anorm);
-    (void)hipFree(m->scale);
     (void)hipFree(m->r);
     (void)hipFree(m->xw);
     (void)hipFree(m->bw);
@@ -173,22 +156,16 @@ static void gemm_f32(lsx_lu* lu, const float* A, int64_t lda, const float* B, in
     lu->prof.end(lu->cur, pi);
 }
 
-// Trailing update of the mixed factorization on BF16 MFMA: FP32 factors with products of three
-// BF16 pieces per operand (FP32 accuracy), or BF16 factors (one piece, per-column power-of-two
-// scales of U12).
+// Trailing update of the mixed factorization on BF16 MFMA: FP32 factors with the products of
+// three BF16 pieces per operand (bf16x6, FP32 accuracy), BF16 factors with two pieces per
+// operand (bf16x3, relative error about 3 2^-18 per product).
 static void gemm_lowp(lsx_lu* lu, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                       int64_t M, int64_t N, int64_t K) {
     if (M <= 0 || N <= 0 || K <= 0) return;
-    const bool bf16 = lu->dtype == LSX_BF16;
-    float* sc = nullptr;
-    if (bf16) {
-        sc = ms(lu)->scale + (lu->cur == lu->stream2 ? lu->n + 64 : 0);  // one set per getrf stream
-        hipLaunchKernelGGL(k_colmax_scale, dim3((unsigned)cdiv(N, 256)), dim3(256), 0, lu->cur, B, ldb, (int)K, (int)N, sc);
-    }
     const bool big = cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)lu->ctx->num_cus;
     const int pi = lu->prof.begin(lu->cur, big ? PK_GEMM : PK_GEMM_SMALL, 2.0 * M * N * K, sizeof(float) * (2.0 * M * N));
-    if (bf16) launch_gemm_split<1>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K, sc);
-    else launch_gemm_split<3>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K, nullptr);
+    if (lu->dtype == LSX_BF16) launch_gemm_split<2>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K);
+    else launch_gemm_split<3>(lu->cur, A, lda, B, ldb, C, ldc, M, N, K);
     lu->prof.end(lu->cur, pi);
 }
 

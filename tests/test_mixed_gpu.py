@@ -37,8 +37,9 @@ def test_random_reaches_fp64(lsx, precision, refinement, n):
 @pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
 def test_conditioned(lsx, cond):
     # A = Q1 diag(s) Q2^T.  FP32 factors refine to FP64 by IR up to cond 1e6 (u32 * cond < 1).
-    # BF16 factors (u = 2^-8) precondition GMRES-IR well only while u * cond < 1; beyond that the
-    # 'auto' mode escalates to FP32 factors, and must still reach 1e-13.
+    # BF16 factors (two-piece BF16 products, u ~ 2^-17) refine by IR while u * cond is small and
+    # precondition GMRES-IR beyond; where that stalls the 'auto' mode escalates to FP32 factors,
+    # and must still reach 1e-13.
     n = 768
     A = conditioned(n, cond, seed=1)
     b = np.random.default_rng(2).standard_normal(n)
@@ -143,8 +144,9 @@ def test_factor_scaled_residual(lsx, precision, n):
     """The low-precision factors satisfy the scaled-residual test of the reference's
     test/Core/blocked_lufact.jl:11-28, ||PA - LU||_1 / (||L||_1 ||U||_1 n u) < 20, with u the unit
     roundoff the factorization promises: FP32's 2^-24 for FP32 factors (their trailing products
-    come from three BF16 pieces per operand, which must stay below that), BF16's 2^-8 for BF16
-    factors.  A dropped or mis-scaled piece would leave FP32 factors at about 2^-16."""
+    come from three BF16 pieces per operand, which must stay below that), 2^-16 for BF16 factors
+    (two pieces per operand: 3 2^-18 per product).  A dropped piece would leave FP32 factors at
+    about 2^-16 and BF16 factors at about 2^-8."""
     rng = np.random.default_rng(n + (0 if precision == "fp32" else 1))
     A = rng.standard_normal((n, n))
     dt = lsx.LSX_F32 if precision == "fp32" else lsx.LSX_BF16
@@ -158,7 +160,7 @@ def test_factor_scaled_residual(lsx, precision, n):
         p = ipiv[k] - 1
         if p != k:
             PA[[k, p], :] = PA[[p, k], :]
-    u = 2.0 ** -24 if precision == "fp32" else 2.0 ** -8
+    u = 2.0 ** -24 if precision == "fp32" else 2.0 ** -16
     res = np.abs(PA - Lf @ Uf).sum(axis=0).max() / (np.abs(Lf).sum(axis=0).max() * np.abs(Uf).sum(axis=0).max() * n * u)
     # and against LAPACK's FP32 sgetrf of the same matrix: the relative factorization error
     # ||PA - LU||_F / ||A||_F of FP32 factors must be of sgetrf's size (products of two BF16 pieces
