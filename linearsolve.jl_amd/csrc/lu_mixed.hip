@@ -60,7 +60,7 @@ __global__ void k_axpy_cols(double* __restrict Y, int64_t ldy, const double* __r
 }
 
 // Partial dot products of columns 0..k-1 of V (n x k, ld) with w, per 256-row block:
-// part[blk * k + j].  Deterministic: the host sums the partials in block order.
+// part[blk * k + j] (k_part_sum adds them up in a fixed order).
 __global__ __launch_bounds__(256) void k_vtw(const double* __restrict V, int64_t ld, int k, const double* __restrict w,
                                              int64_t n, double* __restrict part) {
     __shared__ double red[4][64];
@@ -310,24 +310,31 @@ static int residual(lsx_lu* lu, int64_t c0, int nr, const double* bnorm, double*
     return LSX_OK;
 }
 
-// Deterministic dot products of V's first k columns with w (host sums block partials in order).
-static int vtw(lsx_lu* lu, const double* V, int k, const double* w, double* h) {
-    lsx_ctx* ctx = lu->ctx;
+// out[j] = sum of the block partials part[b k + j] over blocks b (k workgroups of one wave: each
+// lane sums its blocks in order, then a fixed shuffle tree; the same order on every call).
+__global__ __launch_bounds__(64) void k_part_sum(const double* __restrict part, int64_t nb, int k,
+                                                 double* __restrict out) {
+    const int j = blockIdx.x, l = threadIdx.x;
+    double s = 0.0;
+    for (int64_t b = l; b < nb; b += 64) s += part[b * k + j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (l == 0) out[j] = s;
+}
+
+// Deterministic dot products of V's first k columns with w into the device array h (no host
+// round trip).
+static int vtw_dev(lsx_lu* lu, const double* V, int k, const double* w, double* h) {
     MixedState* m = ms(lu);
     const int64_t n = lu->n, nb = cdiv(n, 256);
     hipLaunchKernelGGL(k_vtw, dim3((unsigned)nb), dim3(256), 0, lu->stream, V, n, k, w, n, m->part);
-    LSX_HIP(ctx, hipMemcpyAsync(m->hpart, m->part, sizeof(double) * nb * k, hipMemcpyDeviceToHost, lu->stream));
-    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
-    for (int j = 0; j < k; ++j) {
-        double s = 0.0;
-        for (int64_t b = 0; b < nb; ++b) s += m->hpart[b * k + j];
-        h[j] = s;
-    }
-    return LSX_OK;
+    hipLaunchKernelGGL(k_part_sum, dim3((unsigned)k), dim3(64), 0, lu->stream, (const double*)m->part, nb, k, h);
+    return hipGetLastError() == hipSuccess ? LSX_OK : fail(lu->ctx, LSX_EHIP, "GMRES-IR: dot-product launch failed");
 }
 
 // Solves A d = r (column s of m->r, result into m->r column s) by GMRES in FP64 with the
-// low-precision LU as left preconditioner and classical Gram-Schmidt applied twice.
+// low-precision LU as left preconditioner and classical Gram-Schmidt applied twice.  The dot
+// products and the updates stay on the device; each Arnoldi step copies its 2k + 3 reduced
+// values back once, for the Givens rotations and the stopping test on the host.
 // Returns the number of iterations or < 0.
 static int gmres_ir_correction(lsx_lu* lu, int64_t s, int maxit, double rtol) {
     lsx_ctx* ctx = lu->ctx;
@@ -340,24 +347,30 @@ static int gmres_ir_correction(lsx_lu* lu, int64_t s, int maxit, double rtol) {
         (void)hipFree(m->hvec);
         LSX_HIP(ctx, hipMalloc(&m->V, sizeof(double) * n * (M + 1)));
         LSX_HIP(ctx, hipMalloc(&m->w, sizeof(double) * n));
-        LSX_HIP(ctx, hipMalloc(&m->hvec, sizeof(double) * (M + 2)));
+        LSX_HIP(ctx, hipMalloc(&m->hvec, sizeof(double) * 3 * (M + 2)));
         m->gm = M;
     }
+    // device scalars: hA (first Gram-Schmidt pass), hB (second), hN (||w||^2), one copy per step
+    double* hA = m->hvec;
+    double* hB = hA + (M + 2);
+    double* hN = hB + (M + 2);
     double* r = m->r + s * n;
     const unsigned g = (unsigned)cdiv(n, 256);
     // z0 = M^-1 r
     int rc = mixed_getrs_dev(lu, 1, r, n, m->V, n);
     if (rc) return rc;
     double beta;
-    rc = vtw(lu, m->V, 1, m->V, &beta);
+    rc = vtw_dev(lu, m->V, 1, m->V, hN);
     if (rc) return rc;
+    LSX_HIP(ctx, hipMemcpyAsync(&beta, hN, sizeof(double), hipMemcpyDeviceToHost, lu->stream));
+    LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
     beta = std::sqrt(beta);
     if (beta == 0.0) {
         LSX_HIP(ctx, hipMemsetAsync(r, 0, sizeof(double) * n, lu->stream));
         return 0;
     }
     hipLaunchKernelGGL(k_scale_copy, dim3(g), dim3(256), 0, lu->stream, m->V, (const double*)m->V, 1.0 / beta, n);
-    std::vector<double> H((size_t)(M + 1) * M, 0.0), cs(M), sn(M), z(M + 1, 0.0), h(M + 1), h2(M + 1);
+    std::vector<double> H((size_t)(M + 1) * M, 0.0), cs(M), sn(M), z(M + 1, 0.0), h(M + 1), hh(3 * (M + 2));
     z[0] = beta;
     int k = 0;
     for (; k < M; ++k) {
@@ -365,22 +378,21 @@ static int gmres_ir_correction(lsx_lu* lu, int64_t s, int maxit, double rtol) {
         resid_launch(lu->stream, lu->F, lu->ld, n, m->V + k * n, n, 1, nullptr, 0, m->w, n, m->part, RES_PRODUCT);
         rc = mixed_getrs_dev(lu, 1, m->w, n, m->w, n);
         if (rc) return rc;
-        // CGS2
-        rc = vtw(lu, m->V, k + 1, m->w, h.data());
+        // CGS2 on the device
+        rc = vtw_dev(lu, m->V, k + 1, m->w, hA);
         if (rc) return rc;
-        LSX_HIP(ctx, hipMemcpyAsync(m->hvec, h.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, lu->stream));
         hipLaunchKernelGGL(k_wmvh, dim3(g), dim3(256), 0, lu->stream, m->w, (const double*)m->V, n, k + 1,
-                           (const double*)m->hvec, n);
-        rc = vtw(lu, m->V, k + 1, m->w, h2.data());
+                           (const double*)hA, n);
+        rc = vtw_dev(lu, m->V, k + 1, m->w, hB);
         if (rc) return rc;
-        LSX_HIP(ctx, hipMemcpyAsync(m->hvec, h2.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, lu->stream));
         hipLaunchKernelGGL(k_wmvh, dim3(g), dim3(256), 0, lu->stream, m->w, (const double*)m->V, n, k + 1,
-                           (const double*)m->hvec, n);
-        for (int j = 0; j <= k; ++j) h[j] += h2[j];
-        double hn;
-        rc = vtw(lu, m->w, 1, m->w, &hn);
+                           (const double*)hB, n);
+        rc = vtw_dev(lu, m->w, 1, m->w, hN);
         if (rc) return rc;
-        hn = std::sqrt(hn);
+        LSX_HIP(ctx, hipMemcpyAsync(hh.data(), hA, sizeof(double) * 3 * (M + 2), hipMemcpyDeviceToHost, lu->stream));
+        LSX_HIP(ctx, hipStreamSynchronize(lu->stream));
+        for (int j = 0; j <= k; ++j) h[j] = hh[j] + hh[(M + 2) + j];
+        const double hn = std::sqrt(hh[2 * (M + 2)]);
         // Givens on column k of H
         for (int j = 0; j < k; ++j) {
             const double t = cs[j] * h[j] + sn[j] * h[j + 1];
