@@ -515,7 +515,22 @@ __global__ void k_spmv(int64_t N, const int32_t* __restrict rowptr, const int32_
 
 // Rows per thread for a matrix of N rows: a single workgroup up to 8192 rows, then 16 rows per
 // thread over ceil(N / 8192) workgroups.
+// Rows per thread and workgroups of a solve of N rows: beyond 8192 rows, the smallest of 4, 8, 16
+// rows per thread that keeps the solve within GMRES_WIDE workgroups, so that more CUs stream each
+// basis vector (measured: a 256^2 grid 105 -> 69 us per Arnoldi step with 32 workgroups of 2048
+// rows instead of 8 of 8192).  The bound keeps every solve inside the serving lanes' share of the
+// CUs, so a solve has the same shape, and the same bits, in every launch form.
+constexpr int GMRES_WIDE = 128;
 static int pick_rpt(int64_t N, int& P) {
+    if (N > 16 * GBS) {
+        for (int r : {4, 8}) {
+            const int64_t p = cdiv(N, (int64_t)r * GBS);
+            if (p <= GMRES_WIDE) {
+                P = (int)p;
+                return r;
+            }
+        }
+    }
     if (N <= GBS) { P = 1; return 1; }
     if (N <= 4 * GBS) { P = 1; return 4; }
     if (N <= 16 * GBS) { P = 1; return 16; }
@@ -574,6 +589,7 @@ int gmres_launch_multi(lsx_ctx* ctx, hipStream_t st, const GmresArgs* items, con
     switch (rpt) {
         case 1: hipLaunchKernelGGL(k_gmres_multi<1>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
         case 4: hipLaunchKernelGGL(k_gmres_multi<4>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
+        case 8: hipLaunchKernelGGL(k_gmres_multi<8>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
         default: hipLaunchKernelGGL(k_gmres_multi<16>, dim3(pool), dim3(GBS), dynlds, st, items, first, nitems, ticket); break;
     }
     LSX_HIP(ctx, hipGetLastError());
@@ -645,6 +661,10 @@ int gmres_launch(lsx_csr* A, const GmresParams& p, const double* db, double* dx,
         case 4:
             if (plan.rglob) hipLaunchKernelGGL((k_gmres<4, true>), dim3(P), dim3(GBS), dynlds, st, g);
             else hipLaunchKernelGGL((k_gmres<4, false>), dim3(P), dim3(GBS), dynlds, st, g);
+            break;
+        case 8:
+            if (plan.rglob) hipLaunchKernelGGL((k_gmres<8, true>), dim3(P), dim3(GBS), dynlds, st, g);
+            else hipLaunchKernelGGL((k_gmres<8, false>), dim3(P), dim3(GBS), dynlds, st, g);
             break;
         default:
             if (plan.rglob) hipLaunchKernelGGL((k_gmres<16, true>), dim3(P), dim3(GBS), dynlds, st, g);

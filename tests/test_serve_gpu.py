@@ -157,7 +157,7 @@ def test_serving_equals_direct_solvers(lsx):
     # must equal LUFactorization / SimpleGMRES solves of the same inputs bit for bit.
     from linearsolve_jl_amd import problems as P
     dense = [P.dense_block(n, seed=n + 3) for n in (1500, 600, 900)]  # handle grows, then reshapes
-    grids = [(100, 3.0), (140, 1.0), (60, 2.0), (130, 0.5)]             # 2, 3, 1, 3 workgroups
+    grids = [(100, 3.0), (140, 1.0), (60, 2.0), (130, 0.5)]             # 5, 10, 1, 9 workgroups
     sparse = []
     for m, pe in grids:
         rp, ci, v = P.convection_diffusion(m, pe)
