@@ -124,20 +124,30 @@ __device__ __forceinline__ uint32_t max8_u32(const uint32_t* s) {
 // Payloads of the flag-in-word messages addr(i) selected by mask (i < N <= 64) once they carry
 // sequence number seq: all are read, then the stale ones again until none is left, the bound is
 // reached or a wait of the launch has given up (*abort, shared by the workgroup, stops its later
-// waits at once).  Returns false when it gave up.
+// waits at once).  Returns false when it gave up.  The reads go in groups of 16, every read of a
+// group issued before any is tested (a test right after each read would wait for the reads one
+// at a time; all N at once would hold too many registers).
 template <int N, typename F>
 __device__ __forceinline__ bool calu_poll(F addr, uint64_t mask, uint32_t seq, uint32_t (&out)[N], const CaluArgs& a,
                                           int* abort) {
+    constexpr int PB = 16;
     uint64_t stale = mask;
     int64_t t0 = 0;
     bool first = true;
     while (stale) {
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            if (stale & (1ull << i)) {
-                const uint64_t x = ld_relaxed_u64(addr(i));
-                if ((uint32_t)(x >> 32) == seq) {
-                    out[i] = (uint32_t)x;
+        for (int i0 = 0; i0 < N; i0 += PB) {
+            uint64_t x[PB];
+#pragma unroll
+            for (int j = 0; j < PB; ++j) {
+                const int i = i0 + j;
+                x[j] = (i < N && (stale & (1ull << i))) ? ld_relaxed_u64(addr(i < N ? i : 0)) : 0ull;
+            }
+#pragma unroll
+            for (int j = 0; j < PB; ++j) {
+                const int i = i0 + j;
+                if (i < N && (stale & (1ull << i)) && (uint32_t)(x[j] >> 32) == seq) {
+                    out[i] = (uint32_t)x[j];
                     stale &= ~(1ull << i);
                 }
             }
@@ -259,6 +269,10 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
     __shared__ int s_lab[CW];
     __shared__ int s_slot[CW];       // deciding workgroup: candidate slot of each pivot
     __shared__ int s_pick[CW];       // this workgroup's candidates, in order: their threads
+    // this workgroup's own candidate records (as published): the deciding workgroup, the last to
+    // arrive, reads its own slots here instead of waiting for its stores to come back from memory
+    __shared__ uint32_t s_own[1 + CW][CW];
+    __shared__ float s_ownrem[CW][CREM];
     __shared__ int s_allab[NBO];     // the block's pivot rows, panel after panel
     __shared__ int s_final, s_abort;
     const int np = (a.JB + CW - 1) / CW;
@@ -357,6 +371,7 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
                         const uint32_t pl = q == 0 ? (k < cnt ? (uint32_t)(a.jo + (int64_t)w * CBS + s_pick[k]) : 0xFFFFFFFFu)
                                                    : __float_as_uint(x[i]);
                         st_relaxed_u64(candp + (size_t)q * CSL + w * CW + k, ll_pack(pl, seq));
+                        s_own[q][k] = pl;
                     }
                 }
                 constexpr int NRW = (CW * CREM + CBS - 1) / CBS;
@@ -373,8 +388,10 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
 #pragma unroll
                 for (int i = 0; i < NRW; ++i) {
                     const int idx = tid + i * CBS, k = idx / CREM, j = idx % CREM;
-                    if (k < cnt && j < rem)
+                    if (k < cnt && j < rem) {
                         st_relaxed_u64(candp + CREMW + (size_t)(w * CW + k) * CREM + j, ll_pack(__float_as_uint(y[i]), seq));
+                        s_ownrem[k][j] = y[i];
+                    }
                 }
             }
             if (tid == 0) s_final = (arrived == a.cbase + (uint32_t)(p * a.P + a.P - 1)) ? 1 : 0;
@@ -394,9 +411,14 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
 #pragma unroll
                 for (int q = 0; q < CW; ++q) f[h][q] = 0.f;
                 if (sl / CW < a.P) {
+                    const bool own = sl / CW == w;
                     uint32_t x[1 + CW];
-                    if (calu_poll<1 + CW>([&](int i) { return candp + (size_t)i * CSL + sl; }, (1ull << (1 + CW)) - 1, seq,
-                                          x, a, &s_abort)) {
+                    if (calu_poll<1 + CW>([&](int i) { return candp + (size_t)i * CSL + sl; },
+                                          own ? 0ull : (1ull << (1 + CW)) - 1, seq, x, a, &s_abort)) {
+                        if (own) {
+#pragma unroll
+                            for (int i = 0; i < 1 + CW; ++i) x[i] = s_own[i][sl % CW];
+                        }
                         fl[h] = (int)x[0];
 #pragma unroll
                         for (int q = 0; q < CW; ++q) f[h][q] = __uint_as_float(x[1 + q]);
@@ -488,12 +510,16 @@ __global__ __launch_bounds__(CBS) void k_block_calu(CaluArgs a) {
                 for (int c = 0; c < CW; ++c) {
                     x[c] = 0u;
                     const int sl = s_slot[c];
-                    if (c < W && sl >= 0 && sl < a.P * CW) m |= 1ull << c;
+                    if (c < W && sl >= 0 && sl < a.P * CW && sl / CW != w) m |= 1ull << c;
                 }
                 (void)calu_poll<CW>([&](int c) { return candp + CREMW + (size_t)s_slot[c] * CREM + tid; }, m, seq, x, a,
                                     &s_abort);
 #pragma unroll
-                for (int c = 0; c < CW; ++c) u[c] = (m >> c) & 1 ? __uint_as_float(x[c]) : 0.f;
+                for (int c = 0; c < CW; ++c) {
+                    const int sl = s_slot[c];
+                    const bool own = c < W && sl >= 0 && sl / CW == w;
+                    u[c] = (m >> c) & 1 ? __uint_as_float(x[c]) : (own ? s_ownrem[sl % CW][tid] : 0.f);
+                }
 #pragma unroll
                 for (int c = 1; c < CW; ++c) {  // row c of L11 (contiguous in LDS) against u[0..c)
 #pragma unroll
