@@ -214,6 +214,26 @@ def test_config2_full_size_pivots_bit_exact(lsx, n):
     assert mism.size == 0, f"pivot mismatch at columns {mism[:10]}"
 
 
+def test_beyond_the_partitioned_streams(lsx):
+    # n = 17000: more than 32 panel workgroups, so the panel and the updates share plain streams
+    # (no CU partition), and the last block is partial (17000 = 132 * 128 + 104).  Pivots
+    # bit-exact vs LAPACK, backward error <= 1e-13.
+    n = 17000
+    print(f"n={n}", flush=True)
+    rng = np.random.default_rng(np.random.SeedSequence([5, n]))
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 2))
+    h, info, F, ipiv = factor(lsx, A)
+    assert info == 0
+    X = np.empty((n, 2), order="F")
+    h.solve(B, X)
+    assert backward_error(A, X, B) <= 1e-13
+    del F
+    lu, piv, il = lapack.dgetrf(A, overwrite_a=True)
+    mism = np.nonzero(ipiv != piv + 1)[0]
+    assert mism.size == 0, f"pivot mismatch at columns {mism[:10]}"
+
+
 # ---- adjoint solves: solve!(cache; adjoint = true) reuses the factorization (getrs 'T')
 
 @pytest.mark.parametrize("n", [1, 7, 64, 65, 300, 1000, 2049])

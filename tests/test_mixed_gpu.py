@@ -138,6 +138,21 @@ def test_conditioned_full_size(lsx, cond):
     torch.cuda.empty_cache()
 
 
+def test_beyond_tournament_blocks(lsx):
+    # n = 17000: the first block columns have more rows than one tournament launch holds (16384)
+    # and take the exact panels; the rest tournament pivoting; plain streams (no CU partition).
+    # Refined to a backward error <= 1e-13 with FP32 and BF16 factors.
+    n = 17000
+    rng = np.random.default_rng(17)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    for precision in ("fp32", "bf16"):
+        sol = lsx.solve(lsx.LinearProblem(A, b), lsx.MI355XMixedLUFactorization(precision=precision, tol=1e-13))
+        print(f"n={n} {precision}: {sol.stats}", flush=True)
+        assert sol.retcode == lsx.ReturnCode.Success
+        assert backward_error(A, sol.u, b) <= 1e-13
+
+
 @pytest.mark.parametrize("n", [200, 1000, 2500])
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_factor_scaled_residual(lsx, precision, n):
