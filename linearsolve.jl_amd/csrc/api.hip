@@ -67,6 +67,7 @@ int lu64_alloc(lsx_lu* lu) {
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_start, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_next, hipEventDisableTiming));
     LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_panel, hipEventDisableTiming));
+    LSX_HIP(ctx, hipEventCreateWithFlags(&lu->ev_perm, hipEventDisableTiming));
     lu->cur = lu->stream;
     if (lu->own_F && hipMalloc(&lu->F, sizeof(double) * pad_ld(n) * n) != hipSuccess)
         return fail(ctx, LSX_ENOMEM, "lsx_lu_create: cannot allocate %lld x %lld FP64 factors", (long long)n,
@@ -125,6 +126,7 @@ void lu64_free(lsx_lu* lu) {
     if (lu->ev_start) (void)hipEventDestroy(lu->ev_start);
     if (lu->ev_next) (void)hipEventDestroy(lu->ev_next);
     if (lu->ev_panel) (void)hipEventDestroy(lu->ev_panel);
+    if (lu->ev_perm) (void)hipEventDestroy(lu->ev_perm);
     if (lu->stream2 && lu->stream2 != lu->stream) (void)hipStreamDestroy(lu->stream2);
     if (lu->stream_upd) (void)hipStreamDestroy(lu->stream_upd);
     if (lu->stream) (void)hipStreamDestroy(lu->stream);

@@ -28,7 +28,8 @@ struct lsx_lu {
     hipEvent_t ev_start = nullptr;  // getrf input ready on stream
     hipStream_t cur = nullptr;      // stream the launch helpers use at the moment
     hipEvent_t ev_next = nullptr;   // next block's columns updated (stream -> stream2)
-    hipEvent_t ev_panel = nullptr;  // next block's panel factored (stream2 -> stream)
+    hipEvent_t ev_panel = nullptr;  // next block's panel factored, inverses and row map made (stream2 -> stream)
+    hipEvent_t ev_perm = nullptr;   // next block's interchange list made (stream2 -> stream)
     // FP64 factors (also the FP64 copy of A for the mixed path's residuals)
     double* F = nullptr;
     int64_t* ipiv = nullptr;     // device, 1-based

@@ -89,12 +89,14 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
     // factorization of outer block [jo, jo + jbo) (its columns already updated): inner panels,
     // their interchanges and updates inside the block, diagonal-block inverses, the block's
     // interchange list (parity par) and the row map
-    auto panel_block = [&](int64_t jo, int jbo, int par) {
+    auto panel_block = [&](int64_t jo, int jbo, int par) -> int {
         if (block_fact(jo, jbo)) {
-            // rows were factored in place: the block's interchanges on its own columns, then the
-            // inverses of its diagonal blocks (in the final row order)
+            // rows were factored in place: the block's interchange list (from here on the update
+            // stream may apply it to the columns right of the block), its interchanges on its own
+            // columns, then the inverses of its diagonal blocks (in the final row order)
             hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
                                lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);
+            LSX_HIP(ctx, hipEventRecord(lu->ev_perm, lu->cur));
             launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, jo,
                               jo + jbo, 0);
             const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
@@ -104,7 +106,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
             if (defer)
                 hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
                                    lu->pinv + (jo / NBO) * n, n);
-            return;
+            return LSX_OK;
         }
         for (int64_t ji = jo; ji < jo + jbo; ji += PW) {
             const int W = (int)std::min<int64_t>(PW, jo + jbo - ji);
@@ -121,21 +123,28 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
                 }
             }
         }
+        // the block's interchange list first: the update stream applies it to the columns right
+        // of the block while the diagonal-block inverses are formed
+        hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
+                           lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);
+        LSX_HIP(ctx, hipEventRecord(lu->ev_perm, lu->cur));
         const int64_t b0 = jo / DB, nbk = cdiv(jbo, DB);
         const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
         hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(256), 0, lu->cur, F, ld, n, b0, DL, DU);
         lu->prof.end(lu->cur, pi);
-        hipLaunchKernelGGL(k_perm_build, dim3(1), dim3(256), 0, lu->cur, lu->ipiv, jo, jbo,
-                           lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, lu->perm);
         if (defer)
             hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
                                lu->pinv + (jo / NBO) * n, n);
+        return LSX_OK;
     };
-    // interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 of block [jo, jo + jbo) on columns [c0, c1)
-    auto update = [&](int64_t jo, int jbo, int par, int64_t c0, int64_t c1) {
-        if (c1 <= c0) return;
+    // interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 of block [jo, jo + jbo) on columns
+    // [c0, c1); wait_inv: the block's diagonal-block inverses may still be in the making on the
+    // panel stream (ev_panel), so the products wait for them after the interchanges
+    auto update = [&](int64_t jo, int jbo, int par, int64_t c0, int64_t c1, bool wait_inv) -> int {
+        if (c1 <= c0) return LSX_OK;
         const int64_t r1 = jo + jbo, nc = c1 - c0;
         launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, c0, c1, 0);
+        if (wait_inv && lu->cur != s1) LSX_HIP(ctx, hipStreamWaitEvent(lu->cur, lu->ev_panel, 0));
         // L11 = [L1 0; L2 L3] (64-blocks): X1 = L1^{-1} A12[0:64];  A12[64:] -= L2 X1;  X2 = L3^{-1} A12[64:]
         T* A12 = F + c0 * ld + jo;
         const int64_t b0 = jo / DB;
@@ -146,9 +155,12 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
             gemm_trsm(DL + (b0 + 1) * DB * DB, DB, A12 + DB, ld, A12 + DB, ld, h2, nc, h2, 1);
         }
         gemm_upd(F + jo * ld + r1, ld, A12, ld, F + c0 * ld + r1, ld, n - r1, nc, jbo, 0);
+        return LSX_OK;
     };
 
-    panel_block(0, (int)std::min<int64_t>(NBO, n), 0);
+    int rc_ = panel_block(0, (int)std::min<int64_t>(NBO, n), 0);
+    if (rc_) return rc_;
+    LSX_HIP(ctx, hipEventRecord(lu->ev_panel, s0));
     int par = 0;
     for (int64_t jo = 0; jo < n; jo += NBO, par ^= 1) {
         const int jbo = (int)std::min<int64_t>(NBO, n - jo);
@@ -158,17 +170,20 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
             launch_laswp_t<T>(lu, F, ld, jo, jbo, lu->plist + (4 + 4 * par) * NBO, lu->pcount + 1 + par, 0, jo, 0);
         if (c1 >= n) break;
         const int jbn = (int)std::min<int64_t>(NBO, n - c1);
-        update(jo, jbo, par, c1, c1 + jbn);
+        if ((rc_ = update(jo, jbo, par, c1, c1 + jbn, true))) return rc_;
         LSX_HIP(ctx, hipEventRecord(lu->ev_next, s0));
         LSX_HIP(ctx, hipStreamWaitEvent(s1, lu->ev_next, 0));
         lu->cur = s1;
-        panel_block(c1, jbn, par ^ 1);
+        if ((rc_ = panel_block(c1, jbn, par ^ 1))) return rc_;
         LSX_HIP(ctx, hipEventRecord(lu->ev_panel, s1));
         lu->cur = s0;
-        update(jo, jbo, par, c1 + jbn, n);
-        LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_panel, 0));
+        if ((rc_ = update(jo, jbo, par, c1 + jbn, n, false))) return rc_;
+        // the next block's interchange list (ev_perm) is all the next iteration's first step
+        // needs; its products wait for the inverses (ev_panel) themselves
+        LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_perm, 0));
     }
     lu->cur = s0;
+    LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_panel, 0));  // the last block's inverses and row map
     if (defer && n > NBO) {
         const int64_t ncols = (cdiv(n, NBO) - 1) * NBO;
         const size_t lds = sizeof(T) * (size_t)(n - NBO);
