@@ -501,10 +501,21 @@ __global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict A, int64_t ld,
     __syncthreads();
     const int64_t cb = c0 + (int64_t)blockIdx.x * LCW;
     const int ncol = (int)min((int64_t)LCW, c1 - cb);
-    // gather: consecutive threads walk consecutive list entries of one column
-    for (int idx = t; idx < NT * LCW; idx += 256) {
-        const int cc = idx / NT, i = idx - cc * NT;
-        if (cc < ncol) s_buf[i][cc] = A[(cb + cc) * ld + s_src[i]];
+    // gather: consecutive threads walk consecutive list entries of one column; eight loads of a
+    // thread in flight before their LDS stores (a store right after each load would wait for
+    // the scattered rows one at a time)
+    for (int base = t; base < NT * LCW; base += 8 * 256) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * 256, cc = idx / NT, i = idx - cc * NT;
+            v[u] = (idx < NT * LCW && cc < ncol) ? A[(cb + cc) * ld + s_src[i]] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int idx = base + u * 256, cc = idx / NT, i = idx - cc * NT;
+            if (idx < NT * LCW && cc < ncol) s_buf[i][cc] = v[u];
+        }
     }
     __syncthreads();
     const int first = trsm ? kb : 0;
