@@ -462,10 +462,28 @@ __global__ __launch_bounds__(256) void k_left_gather(T* __restrict F, int64_t ld
     const int64_t r0 = (b + 1) * NBO;
     if (r0 >= n) return;
     T* col = F + c * ld;
-    for (int64_t i = r0 + threadIdx.x; i < n; i += 256) s[i - r0] = col[i];
+    // eight loads of a thread in flight before their LDS stores, and eight row indices (two
+    // dependent loads each) before the gathers
+    for (int64_t i0 = r0 + threadIdx.x; i0 < n; i0 += 8 * 256) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = i0 + u * 256 < n ? col[i0 + u * 256] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u * 256 < n) s[i0 + u * 256 - r0] = v[u];
+    }
     __syncthreads();
     const int* inv = pinv + b * n;
-    for (int64_t i = r0 + threadIdx.x; i < n; i += 256) col[i] = s[inv[perm[i]] - r0];
+    for (int64_t i0 = r0 + threadIdx.x; i0 < n; i0 += 8 * 256) {
+        int pr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pr[u] = i0 + u * 256 < n ? perm[i0 + u * 256] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pr[u] = i0 + u * 256 < n ? inv[pr[u]] : 0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u * 256 < n) col[i0 + u * 256] = s[pr[u] - r0];
+    }
 }
 
 static __global__ void k_iota(int* __restrict p, int64_t n) {
@@ -561,9 +579,19 @@ __global__ __launch_bounds__(256) void k_diag_inv(const T* __restrict F, int64_t
     const int nb = (int)min((int64_t)DB, n - r0);
     const int t = threadIdx.x, c = t >> 2, p = t & 3;
     __shared__ T D[DB][DB + 1];   // D[i][k]
-    for (int idx = t; idx < DB * DB; idx += 256) {
-        const int i = idx & (DB - 1), k = idx / DB;
-        D[i][k] = (i < nb && k < nb) ? F[(r0 + k) * ld + r0 + i] : (i == k ? T(1) : T(0));
+    {
+        // all sixteen loads of a thread issued before any LDS store
+        T v[DB * DB / 256];
+#pragma unroll
+        for (int u = 0; u < DB * DB / 256; ++u) {
+            const int idx = t + 256 * u, i = idx & (DB - 1), k = idx / DB;
+            v[u] = (i < nb && k < nb) ? F[(r0 + k) * ld + r0 + i] : (i == k ? T(1) : T(0));
+        }
+#pragma unroll
+        for (int u = 0; u < DB * DB / 256; ++u) {
+            const int idx = t + 256 * u;
+            D[idx & (DB - 1)][idx / DB] = v[u];
+        }
     }
     __syncthreads();
     T xk[DB / 4];  // x[4 m + p]
@@ -649,9 +677,15 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict F, int64_t ld,
         if (idx < DB * NR && gr < n && s < nrhs)
             bin[i] = LOWER ? In[(int64_t)s * ldin + perm[gr]] : In[(int64_t)s * ldin + gr];
     }
-    for (int idx = tid; idx < DB * DB; idx += 256) {
-        const int i = idx & (DB - 1), k = idx >> 6;
-        Ds[i][k] = (double)Dinv[(int64_t)bw * DB * DB + k * DB + i];
+    {
+        T dv[DB * DB / 256];  // all loads in flight before the LDS stores
+#pragma unroll
+        for (int u = 0; u < DB * DB / 256; ++u) dv[u] = Dinv[(int64_t)bw * DB * DB + tid + 256 * u];
+#pragma unroll
+        for (int u = 0; u < DB * DB / 256; ++u) {
+            const int idx = tid + 256 * u, i = idx & (DB - 1), k = idx >> 6;
+            Ds[i][k] = (double)dv[u];
+        }
     }
     double acc[NR];
 #pragma unroll
