@@ -231,7 +231,8 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     x = torch.empty_like(b)
     torch.cuda.synchronize()
     log("batched: systems ready, creating the batch handle")
-    table = lsx.load_tile_table()  # autotuned size-binned panel widths (tile_table.json)
+    # autotuned size-binned panel widths per factor type (tile_table.json)
+    tables = {dt: lsx.load_tile_table(dtype=dt) for dt in (_abi.LSX_F64, _abi.LSX_F32)}
     info = np.zeros(count, dtype=np.int64)
     rs = np.random.default_rng(7)
     pick = rs.choice(count, size=min(200, count), replace=False)
@@ -250,6 +251,7 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
     def run(dtype):
         hb = ctypes.c_void_p()
         ctx.check(L.lsx_batch_create(ctx.handle, count, sizes.ctypes.data, dtype, ctypes.byref(hb)), "lsx_batch_create")
+        table = tables[dtype]
         if table:
             up = np.array([u for u, _ in table], dtype=np.int32)
             bw = np.array([w for _, w in table], dtype=np.int32)
@@ -304,8 +306,9 @@ def run_batched(lsx, _abi, ctx, L, count, reps, loop_sample):
                per_system_loop=dict(solves_per_s=round(1.0 / tl, 1), sample=len(loop_idx),
                                     kind="lsx_lu_factor + lsx_lu_solve per system, handles reused"),
                speedup_vs_loop=round((count / tb) * tl, 2),
-               tile_table=table if table else "library default",
+               tile_table=tables[_abi.LSX_F64] or "library default",
                fp32_refined=dict(ms=round(t32 * 1e3, 3), solves_per_s=round(count / t32, 1), failed=int(nf32),
+                                 tile_table=tables[_abi.LSX_F32] or "library default",
                                  refinement_steps_max=int(it32.max()), refinement_steps_mean=round(float(it32.mean()), 2),
                                  backward_error_max=float(be32.max()), backward_error_max_sampled=worst32,
                                  speedup_vs_loop=round((count / t32) * tl, 2)))

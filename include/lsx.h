@@ -181,7 +181,7 @@ int lsx_batch_destroy(lsx_batch* batch);
  * receives the pivots packed like b.  Returns the number of systems with info > 0. */
 int lsx_batch_factor_solve(lsx_batch* batch, const double* A, const double* b, double* x,
                            int64_t* info, int64_t* ipiv);
-/* Size-binned tile table: a system of size n uses panel width bw[k] (8 or 16) of the first bin
+/* Size-binned tile table: a system of size n uses panel width bw[k] (8, 16 or 32) of the first bin
  * with n <= upper[k]; sizes beyond the last bin keep the built-in choice.  Chosen by the autotuner (linearsolve.jl_amd/autotune.py); no reference
  * counterpart. */
 int lsx_batch_set_tiles(lsx_batch* batch, int32_t nbins, const int32_t* upper, const int32_t* bw);

@@ -520,13 +520,16 @@ class BlockDiagonal:
 TILE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_table.json")
 
 
-def load_tile_table(path=TILE_TABLE):
-    """Size-binned panel widths written by autotune.py: {"bins": [[upper, bw], ...]} (None if absent)."""
+def load_tile_table(path=TILE_TABLE, dtype=LSX_F64):
+    """Size-binned panel widths written by autotune.py for the factor type dtype:
+    {"bins": [[upper, bw], ...] (FP64 factors), "bins_fp32": [...] (FP32 factors)} (None if absent;
+    FP32 batches fall back to the FP64 table when the file has none of their own)."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return [(int(u), int(w)) for u, w in d["bins"]]
+    key = "bins_fp32" if dtype == LSX_F32 and "bins_fp32" in d else "bins"
+    return [(int(u), int(w)) for u, w in d[key]]
 
 
 class _BatchHandle:
@@ -537,7 +540,7 @@ class _BatchHandle:
         ctx.check(_abi.lib().lsx_batch_create(ctx.handle, len(self.sizes), self.sizes.ctypes.data, dtype,
                                               ctypes.byref(h)), "lsx_batch_create")
         self.h = h
-        table = load_tile_table() if tiles == "auto" else tiles
+        table = load_tile_table(dtype=dtype) if tiles == "auto" else tiles
         if table:
             self.set_tiles(table)
 

@@ -1,9 +1,13 @@
-"""Autotuner of the batched LU's size-binned tile table.
+"""Autotuner of the batched LU's size-binned tile tables, one per factor type.
 
-For every size bin it builds a sample batch of that bin's sizes (A = I - g J, the batched
-workload's distribution, generated on the GPU), times the persistent batched launch with each
-candidate panel width, keeps the fastest, and writes the table to tile_table.json beside this
-file; _BatchHandle loads that file when it exists (lsx_batch_set_tiles).
+For every factor type (FP64; FP32 with FP64 refinement) and size bin it builds a sample batch of
+that bin's sizes (A = I - g J, the batched workload's distribution, generated on the GPU), times
+the persistent batched launch with each candidate panel width, keeps the fastest, and writes the
+tables to tile_table.json beside this file; _BatchHandle loads the table of its factor type when
+the file exists (lsx_batch_set_tiles).  The panel width fixes the blocking of a system: widths 8
+and 16 hold the panel's L21 in LDS (FP64) for trailing-update tiles dealt to the waves in turn;
+width 32 keeps a row block of L21 in each wave's registers and sweeps its columns, with half the
+trailing-matrix traffic of width 16.
 
     python linearsolve.jl_amd/autotune.py [out.json]   # default: tile_table.json beside this file
 """
@@ -15,7 +19,8 @@ import time
 import numpy as np
 
 DEFAULT_BINS = (32, 64, 128, 256, 384, 512)
-WIDTHS = (8, 16)
+WIDTHS = (8, 16, 32)
+DTYPES = (("bins", "LSX_F64"), ("bins_fp32", "LSX_F32"))
 
 
 def _package():
@@ -51,45 +56,53 @@ def sample_batch(sizes, seed):
     return A, b
 
 
-def tune(bins=DEFAULT_BINS, widths=WIDTHS, per_bin=1024, reps=3, seed=0, path=None, write=True):
-    """Returns [(upper, width), ...] and writes it (with the timings) to path (tile_table.json)."""
+def tune(bins=DEFAULT_BINS, widths=WIDTHS, per_bin=1024, reps=3, seed=0, path=None, write=True, dtypes=DTYPES):
+    """Returns {key: [(upper, width), ...]} per factor type and writes it (with the timings) to
+    path (tile_table.json)."""
     import torch
     lsx = _package()
     from linearsolve_jl_amd import _abi
     ctx = lsx.Context.get(0)
     path = path or lsx.TILE_TABLE
     rng = np.random.default_rng(seed)
-    table, timings = [], {}
-    lo = 1
+    samples, lo = [], 1
     for up in bins:
-        sizes = rng.integers(lo, up + 1, size=per_bin)
-        A, b = sample_batch(sizes, seed + up)
-        x = torch.empty_like(b)
-        h = lsx._BatchHandle(ctx, sizes, _abi.LSX_F64, tiles=None)
-        times = {}
-        for w in widths:
-            h.set_tiles([(up, w)])
-            h.factor_solve(A, b, x)
-            best = float("inf")
-            for _ in range(reps):
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                h.factor_solve(A, b, x)
-                torch.cuda.synchronize()
-                best = min(best, time.perf_counter() - t0)
-            times[w] = best
-        win = min(times, key=times.get)
-        table.append((int(up), int(win)))
-        timings[str(up)] = {str(w): round(t * 1e3, 4) for w, t in times.items()}
-        print(f"bin n <= {up}: " + ", ".join(f"bw {w}: {t * 1e3:.3f} ms" for w, t in times.items())
-              + f" -> {win}", flush=True)
-        del h, A, b, x
+        samples.append((up, rng.integers(lo, up + 1, size=per_bin)))
         lo = up + 1
+    out = {"systems_per_bin": per_bin, "device": torch.cuda.get_device_name(0)}
+    tables = {}
+    for key, dname in dtypes:
+        dtype = getattr(_abi, dname)
+        table, timings = [], {}
+        for up, sizes in samples:
+            A, b = sample_batch(sizes, seed + up)
+            x = torch.empty_like(b)
+            h = lsx._BatchHandle(ctx, sizes, dtype, tiles=None)
+            times = {}
+            for w in widths:
+                h.set_tiles([(up, w)])
+                h.factor_solve(A, b, x)
+                best = float("inf")
+                for _ in range(reps):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    h.factor_solve(A, b, x)
+                    torch.cuda.synchronize()
+                    best = min(best, time.perf_counter() - t0)
+                times[w] = best
+            win = min(times, key=times.get)
+            table.append((int(up), int(win)))
+            timings[str(up)] = {str(w): round(t * 1e3, 4) for w, t in times.items()}
+            print(f"{dname} bin n <= {up}: " + ", ".join(f"bw {w}: {t * 1e3:.3f} ms" for w, t in times.items())
+                  + f" -> {win}", flush=True)
+            del h, A, b, x
+        tables[key] = table
+        out[key] = table
+        out["ms_per_launch" + key[4:]] = timings
     if write:
         with open(path, "w") as f:
-            json.dump({"bins": table, "ms_per_launch": timings, "systems_per_bin": per_bin,
-                       "device": torch.cuda.get_device_name(0)}, f, indent=1)
-    return table
+            json.dump(out, f, indent=1)
+    return tables
 
 
 if __name__ == "__main__":

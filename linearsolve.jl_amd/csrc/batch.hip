@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "wave.hpp"
 
 namespace lsx {
 
@@ -51,10 +52,24 @@ static_assert(RPT <= 2, "a pivot row is held by the first or the last of a threa
 // from the factor in L2 and 32-row solve blocks; the extra systems hide the latency of its
 // refinement passes.  (FP64 at four systems per CU spills registers and runs slower.)
 template <typename T> struct BatchCfg;
-template <> struct BatchCfg<double> { static constexpr int WG = 2, RB = 4, CB = 2, SB = 64; static constexpr bool HOLD = true; };
-template <> struct BatchCfg<float> { static constexpr int WG = 4, RB = 2, CB = 2, SB = 32; static constexpr bool HOLD = false; };
+// Panels of 32 columns never hold L21 (it would take 136 KB of LDS); their trailing update
+// (and every FP32 one) keeps a row block of L21 in registers and sweeps the column blocks of
+// its rows, in RB2 x CB2 tiles.
+// (Loading a wave's next tile while it forms the current one measured slower: the registers
+// it takes cost spills and occupancy.)
+template <> struct BatchCfg<double> {
+    static constexpr int WG = 2, RB = 4, CB = 2, RB2 = 2, CB2 = 2, SB = 64;
+    static constexpr bool HOLD = true;
+};
+template <> struct BatchCfg<float> {
+    static constexpr int WG = 4, RB = 2, CB = 2, RB2 = 2, CB2 = 2, SB = 32;
+    static constexpr bool HOLD = false;
+};
+constexpr int PMAX = 32;  // widest panel
+// Whether a panel of width BW holds L21 in LDS and the pivot row in registers.
+template <typename T, int BW> struct PanelHold { static constexpr bool value = BatchCfg<T>::HOLD && BW <= 16; };
 
-// Panel-width index of a system without a tile table.
+// Panel-width index (0: 8, 1: 16, 2: 32 columns) of a system without a tile table.
 __host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : 1; }
 
 struct BatchArgs {
@@ -82,16 +97,18 @@ template <typename T>
 struct BatchShared {
     static constexpr int SB = BatchCfg<T>::SB;
     static_assert(BNMAX <= SB * (SB + 1), "the FP32 residual is kept in the solve staging block");
-    double l11[BatchCfg<T>::HOLD ? 1 : 16][17];  // L11 for U12 (when L21 is held: its first rows)
+    static_assert(PMAX * (PMAX + 1) <= SB * (SB + 1), "L11 of the widest panel fits the staging block");
     union {
-        double l21[BatchCfg<T>::HOLD ? BNMAX * 17 : 1];  // held L21 (row r of the panel at r * 17)
-        double dblk[SB][SB + 1];   // staged diagonal block of the solves / FP32 path: the residual
+        double l21[BatchCfg<T>::HOLD ? BNMAX * 17 : 1];  // held L21 (row r of the panel at r * 17;
+                                                         // its first rows are L11)
+        double dblk[SB][SB + 1];   // staged diagonal block of the solves / FP32 path: the residual /
+                                   // during the factorization, L11 of a panel without held L21
     };
-    double wkey[BT / 64];
+    uint64_t wkey[BT / 64];   // pivot search: each wave's best key and row
     int wrow[BT / 64];
-    double rowbuf[2][16];      // pivot row and the row it displaces (panel columns)
+    double rowbuf[2][PMAX];    // pivot row and the row it displaces (panel columns)
     double red[2][BT / 64];
-    int chg[16];               // rows below the panel that its interchanges move
+    int chg[PMAX];             // rows below the panel that its interchanges move
     int nchg;
     double xs[BNMAX];          // solve vector
     union {
@@ -107,9 +124,6 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-__device__ __forceinline__ bool bbetter(double ka, int ra, double kb, int rb) {
-    return ka > kb || (ka == kb && ra < rb);
-}
 
 // MFMA of the factor type: the transposed 16 x 16 tile D = X Y (X(i, k) = -U12(k, col i),
 // Y(k, j) = L21(row j, k)); lane l / slot q holds C(row rb + (l & 15), col cb + ccol(l, q)).
@@ -152,31 +166,29 @@ __device__ __forceinline__ void panel_col(BatchShared<T>& sh, T (&v)[RPT][BW], i
     if constexpr (C < BW) {
         const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
         if (C < jb) {  // uniform: only the last panel is narrower
-            // pivot: first row of max |a| among logical rows C..m-1 (NaN never wins)
-            double key = -1.0;
-            int krow = 0x7fffffff;
+            // pivot: first row of max |a| among logical rows C..m-1 (NaN counts as 0), by packed
+            // keys (0: no candidate): the thread's rows, the wave (DPP), then the four waves
+            uint64_t key = 0;
+            int krow = INT_MAX;
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 const int r = tid + k * BT;
                 if (r >= C && r < m) {
-                    const double t = fabs((double)v[k][C]);
-                    const double kk = (t == t) ? t : 0.0;
-                    if (bbetter(kk, r, key, krow)) { key = kk; krow = r; }
+                    const uint64_t u = pivot_key(v[k][C]);
+                    if (u > key) { key = u; krow = r; }  // rows increase with k: a tie keeps the first
                 }
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ok = __shfl_xor(key, off);
-                const int orow = __shfl_xor(krow, off);
-                if (bbetter(ok, orow, key, krow)) { key = ok; krow = orow; }
-            }
-            if (lane == 0) { sh.wkey[wave] = key; sh.wrow[wave] = krow; }
+            const int wl = wave_argmax(key, krow);
+            if (lane == wl) { sh.wkey[wave] = key; sh.wrow[wave] = krow; }
             __syncthreads();
-            double bk = sh.wkey[0];
+            uint64_t bk = sh.wkey[0];
             int pr = sh.wrow[0];
 #pragma unroll
-            for (int i = 1; i < BT / 64; ++i)
-                if (bbetter(sh.wkey[i], sh.wrow[i], bk, pr)) { bk = sh.wkey[i]; pr = sh.wrow[i]; }
+            for (int i = 1; i < BT / 64; ++i) {
+                const uint64_t ki = sh.wkey[i];
+                const int ri = sh.wrow[i];
+                if (ki > bk || (ki == bk && ri < pr)) { bk = ki; pr = ri; }
+            }
             // the pivot row and (when they differ) row C through LDS; row C is thread C's first row
             if (tid == (pr & (BT - 1))) {
 #pragma unroll
@@ -204,8 +216,8 @@ __device__ __forceinline__ void panel_col(BatchShared<T>& sh, T (&v)[RPT][BW], i
             // the pivot row: held in registers, or read from LDS where it is used
             T prh[BW];
 #pragma unroll
-            for (int q = 0; q < BW; ++q) prh[q] = BatchCfg<T>::HOLD ? (T)sh.rowbuf[0][q] : T(0);
-            auto prow = [&](int q) -> T { return BatchCfg<T>::HOLD ? prh[q] : (T)sh.rowbuf[0][q]; };
+            for (int q = 0; q < BW; ++q) prh[q] = PanelHold<T, BW>::value ? (T)sh.rowbuf[0][q] : T(0);
+            auto prow = [&](int q) -> T { return PanelHold<T, BW>::value ? prh[q] : (T)sh.rowbuf[0][q]; };
             if (pr != C) {
                 if (tid == C) {
 #pragma unroll
@@ -252,7 +264,10 @@ __device__ __forceinline__ void panel_col(BatchShared<T>& sh, T (&v)[RPT][BW], i
 // in sh.perm.  On return A holds L and U in LAPACK's row order.
 template <typename T, int BW>
 __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int64_t vo, int64_t* ipiv, int64_t& info) {
+    static_assert(BW == 8 || BW == 16 || BW == PMAX, "panel width");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr bool HL = PanelHold<T, BW>::value;
+    double (*L11)[PMAX + 1] = reinterpret_cast<double (*)[PMAX + 1]>(&sh.dblk[0][0]);
     for (int j0 = 0; j0 < n; j0 += BW) {
         const int jb = min(BW, n - j0);
         const int m = n - j0;
@@ -276,24 +291,24 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
 #pragma unroll
                 for (int q = 0; q < BW; ++q) {
                     if (q < jb) A[(j0 + q) * n + j0 + r] = v[k][q];
-                    if constexpr (BatchCfg<T>::HOLD) sh.l21[r * 17 + q] = (double)v[k][q];
-                    else if (k == 0 && r < BW) sh.l11[r][q] = (double)v[k][q];
+                    if constexpr (HL) sh.l21[r * 17 + q] = (double)v[k][q];
+                    else if (k == 0 && r < BW) L11[r][q] = (double)v[k][q];
                 }
             }
         }
         __syncthreads();
         const int c1 = j0 + jb;
         if (c1 < n) {
-            // the rows below the panel that its swaps move: those it swaps with (at most jb)
-            if (tid == 0) {
-                int cnt = 0;
-                for (int k = j0; k < c1; ++k) {
-                    const int p = sh.piv[k];
-                    bool skip = p < c1;
-                    for (int t = 0; t < cnt && !skip; ++t) skip = sh.chg[t] == p;
-                    if (!skip) sh.chg[cnt++] = p;
-                }
-                sh.nchg = cnt;
+            // the rows below the panel that its swaps move: those it swaps with (at most jb), in
+            // the order of the swaps, each once.  Lane k of wave 0 keeps swap k's row unless an
+            // earlier swap names it; the kept rows are compacted by a ballot.
+            if (wave == 0) {
+                const int p = lane < jb ? sh.piv[j0 + lane] : -1;
+                bool keep = p >= c1;
+                for (int k = 0; k < jb; ++k) keep = keep && !(k < lane && sh.piv[j0 + k] == p);
+                const unsigned long long kept = __ballot(keep);
+                if (keep) sh.chg[__popcll(kept & ((1ull << lane) - 1ull))] = p;
+                if (lane == 0) sh.nchg = __popcll(kept);
             }
             __syncthreads();
             const int nchg = sh.nchg;
@@ -323,7 +338,9 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
                 for (int kk = 0; kk < BW; ++kk) {
 #pragma unroll
                     for (int i = kk + 1; i < BW; ++i) {
-                        const double l = BatchCfg<T>::HOLD ? sh.l21[i * 17 + kk] : sh.l11[BatchCfg<T>::HOLD ? 0 : i][kk];
+                        double l;
+                        if constexpr (HL) l = sh.l21[i * 17 + kk];
+                        else l = L11[i][kk];
                         u[i] = fma(-(T)l, u[kk], u[i]);
                     }
                     __builtin_amdgcn_sched_barrier(0);  // L11 read column by column, not all at once
@@ -332,60 +349,109 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
                 for (int i = 0; i < BW; ++i) A[co + j0 + i] = u[i];
             }
             __syncthreads();
-            constexpr int RBT = BatchCfg<T>::RB, CBT = BatchCfg<T>::CB;
-            // A22 -= L21 U12 on MFMA: blocks of RBT x CBT 16 x 16 tiles, waves round-robin
             const int mr = n - c1;
-            const int tbr = (mr + 16 * RBT - 1) / (16 * RBT), tbc = (mr + 16 * CBT - 1) / (16 * CBT);
             T* A22 = A + c1 * n + c1;
             const T* U12 = A + c1 * n + j0;
             const T* L21 = A + j0 * n + c1;
-            for (int t = wave; t < tbr * tbc; t += BT / 64) {
-                const int rb = (t % tbr) * (16 * RBT), cb = (t / tbr) * (16 * CBT);
-                int lr[RBT];
-                bool rok[RBT];
+            using acc_t = typename BMma<T>::acc_t;
+            constexpr int KS = BW / 4, NW = BT / 64;
+            // C tile (R x C 16 x 16 blocks at rows rb, columns cb of A22) and its U12 operands
+            // (lane l, k-step s: -U12(4 s + l / 16, column cb + 16 b + l % 16))
+            auto load_c = [&](auto& c, int rb, int cb) {
+                constexpr int R = sizeof(c) / sizeof(c[0]), C = sizeof(c[0]) / sizeof(c[0][0]);
 #pragma unroll
-                for (int a2 = 0; a2 < RBT; ++a2) {
-                    lr[a2] = rb + 16 * a2 + (lane & 15);
-                    rok[a2] = lr[a2] < mr;
-                }
-                typename BMma<T>::acc_t acc[RBT][CBT];
+                for (int a2 = 0; a2 < R; ++a2)
 #pragma unroll
-                for (int a2 = 0; a2 < RBT; ++a2)
-#pragma unroll
-                    for (int b2 = 0; b2 < CBT; ++b2)
+                    for (int b2 = 0; b2 < C; ++b2)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const int lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
-                            acc[a2][b2][q] = (rok[a2] && lc < mr) ? A22[lc * n + lr[a2]] : T(0);
+                            const int lr = rb + 16 * a2 + (lane & 15), lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
+                            c[a2][b2][q] = (lr < mr && lc < mr) ? A22[lc * n + lr] : T(0);
                         }
+            };
+            auto load_u = [&](auto& x, int cb) {
+                constexpr int C = sizeof(x) / sizeof(x[0]);
 #pragma unroll
-                for (int k4 = 0; k4 < BW / 4; ++k4) {
-                    const int kk = k4 * 4 + (lane >> 4);
-                    T xv[CBT], yv[RBT];
+                for (int b2 = 0; b2 < C; ++b2) {
+                    const int lc = cb + 16 * b2 + (lane & 15);
 #pragma unroll
-                    for (int b2 = 0; b2 < CBT; ++b2) {
-                        const int lc = cb + 16 * b2 + (lane & 15);
-                        xv[b2] = (lc < mr) ? -U12[lc * n + kk] : T(0);
-                    }
-#pragma unroll
-                    for (int a2 = 0; a2 < RBT; ++a2) {
-                        if constexpr (BatchCfg<T>::HOLD) yv[a2] = rok[a2] ? (T)sh.l21[(BW + lr[a2]) * 17 + kk] : T(0);
-                        else yv[a2] = rok[a2] ? L21[kk * n + lr[a2]] : T(0);
-                    }
-#pragma unroll
-                    for (int a2 = 0; a2 < RBT; ++a2)
-#pragma unroll
-                        for (int b2 = 0; b2 < CBT; ++b2) acc[a2][b2] = BMma<T>::mma(xv[b2], yv[a2], acc[a2][b2]);
+                    for (int k4 = 0; k4 < KS; ++k4) x[b2][k4] = (lc < mr) ? -U12[lc * n + k4 * 4 + (lane >> 4)] : T(0);
                 }
+            };
+            auto store_c = [&](const auto& c, int rb, int cb) {
+                constexpr int R = sizeof(c) / sizeof(c[0]), C = sizeof(c[0]) / sizeof(c[0][0]);
 #pragma unroll
-                for (int a2 = 0; a2 < RBT; ++a2)
+                for (int a2 = 0; a2 < R; ++a2)
 #pragma unroll
-                    for (int b2 = 0; b2 < CBT; ++b2)
+                    for (int b2 = 0; b2 < C; ++b2)
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const int lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
-                            if (rok[a2] && lc < mr) A22[lc * n + lr[a2]] = acc[a2][b2][q];
+                            const int lr = rb + 16 * a2 + (lane & 15), lc = cb + 16 * b2 + BMma<T>::ccol(lane, q);
+                            if (lr < mr && lc < mr) A22[lc * n + lr] = c[a2][b2][q];
                         }
+            };
+            if constexpr (HL) {
+                // A22 -= L21 U12 on MFMA: tiles of RBT x CBT 16 x 16 blocks, waves round-robin,
+                // L21 from LDS
+                constexpr int RBT = BatchCfg<T>::RB, CBT = BatchCfg<T>::CB;
+                const int tbr = (mr + 16 * RBT - 1) / (16 * RBT), tbc = (mr + 16 * CBT - 1) / (16 * CBT);
+                for (int t = wave; t < tbr * tbc; t += NW) {
+                    const int rb = (t % tbr) * (16 * RBT), cb = (t / tbr) * (16 * CBT);
+                    acc_t acc[RBT][CBT];
+                    T xv[CBT][KS];
+                    load_c(acc, rb, cb);
+                    load_u(xv, cb);
+#pragma unroll
+                    for (int k4 = 0; k4 < KS; ++k4) {
+                        const int kk = k4 * 4 + (lane >> 4);
+                        T yv[RBT];
+#pragma unroll
+                        for (int a2 = 0; a2 < RBT; ++a2) {
+                            const int lr = rb + 16 * a2 + (lane & 15);
+                            yv[a2] = lr < mr ? (T)sh.l21[(BW + lr) * 17 + kk] : T(0);
+                        }
+#pragma unroll
+                        for (int a2 = 0; a2 < RBT; ++a2)
+#pragma unroll
+                            for (int b2 = 0; b2 < CBT; ++b2) acc[a2][b2] = BMma<T>::mma(xv[b2][k4], yv[a2], acc[a2][b2]);
+                    }
+                    store_c(acc, rb, cb);
+                }
+            } else {
+                // A22 -= L21 U12 on MFMA by row blocks of 16 R2 rows: a wave loads its row block
+                // of L21 into registers once and sweeps the column blocks (16 C2 columns) of a
+                // segment of A22's columns; with fewer row blocks than waves, the columns are cut
+                // into segments so that every wave has work.  U12 is read per tile (out of L2).
+                constexpr int R2 = BatchCfg<T>::RB2, C2 = BatchCfg<T>::CB2;
+                const int tbr = (mr + 16 * R2 - 1) / (16 * R2), tbc = (mr + 16 * C2 - 1) / (16 * C2);
+                const int nseg = min(tbc, (NW + tbr - 1) / tbr);
+                const int segw = (tbc + nseg - 1) / nseg;
+                for (int u = wave; u < tbr * nseg; u += NW) {
+                    const int rb = (u % tbr) * (16 * R2), sg = u / tbr;
+                    T yl[R2][KS];
+#pragma unroll
+                    for (int a2 = 0; a2 < R2; ++a2) {
+                        const int lr = rb + 16 * a2 + (lane & 15);
+#pragma unroll
+                        for (int k4 = 0; k4 < KS; ++k4) yl[a2][k4] = lr < mr ? L21[(k4 * 4 + (lane >> 4)) * n + lr] : T(0);
+                    }
+                    const int cbs = sg * segw, cbe = min(tbc, (sg + 1) * segw);
+                    for (int cbi = cbs; cbi < cbe; ++cbi) {
+                        const int cb = cbi * (16 * C2);
+                        acc_t acc[R2][C2];
+                        T xv[C2][KS];
+                        load_c(acc, rb, cb);
+                        load_u(xv, cb);
+#pragma unroll
+                        for (int k4 = 0; k4 < KS; ++k4)
+#pragma unroll
+                            for (int a2 = 0; a2 < R2; ++a2)
+#pragma unroll
+                                for (int b2 = 0; b2 < C2; ++b2)
+                                    acc[a2][b2] = BMma<T>::mma(xv[b2][k4], yl[a2][k4], acc[a2][b2]);
+                        store_c(acc, rb, cb);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -400,21 +466,27 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
         const int j0 = p * BW, jb = min(BW, n - j0), c1 = j0 + jb, m2 = n - c1;
         if (moved) {
             // thread tid moves rows c1 + tid + k BT, k < RPT (m2 < RPT BT), of the panel's columns,
-            // through registers: every source is read before any row is written
-            T g[RPT][BW];
+            // through registers, GC columns at a time: in each group every source is read before
+            // any row is written
+            constexpr int GC = BW < 16 ? BW : 16;
 #pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int r = tid + k * BT;
+            for (int cg = 0; cg < BW; cg += GC) {
+                T g[RPT][GC];
 #pragma unroll
-                for (int c = 0; c < BW; ++c) g[k][c] = (r < m2 && c < jb) ? A[(j0 + c) * n + sh.perm[c1 + r]] : T(0);
-            }
-            __syncthreads();
+                for (int k = 0; k < RPT; ++k) {
+                    const int r = tid + k * BT;
 #pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int r = tid + k * BT;
+                    for (int c = 0; c < GC; ++c)
+                        g[k][c] = (r < m2 && cg + c < jb) ? A[(j0 + cg + c) * n + sh.perm[c1 + r]] : T(0);
+                }
+                __syncthreads();
 #pragma unroll
-                for (int c = 0; c < BW; ++c)
-                    if (r < m2 && c < jb) A[(j0 + c) * n + c1 + r] = g[k][c];
+                for (int k = 0; k < RPT; ++k) {
+                    const int r = tid + k * BT;
+#pragma unroll
+                    for (int c = 0; c < GC; ++c)
+                        if (r < m2 && cg + c < jb) A[(j0 + cg + c) * n + c1 + r] = g[k][c];
+                }
             }
         }
         // S := tau(j0) o ... o tau(c1 - 1) o S, tau(k) exchanging rows k and piv[k]
@@ -433,19 +505,57 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
     }
 }
 
+// s - sum_{k0 <= k < k1} a[k * n] x[k] in the order of k (FP64), sixteen loads in flight.
+template <typename T>
+__device__ __forceinline__ double row_dot_sub(double s, const T* a, int n, const double* x, int k0, int k1) {
+    int k = k0;
+    for (; k + 16 <= k1; k += 16) {
+        T av[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) av[u] = a[(int64_t)(k + u) * n];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s = fma(-(double)av[u], x[k + u], s);
+    }
+    for (; k < k1; ++k) s = fma(-(double)a[(int64_t)k * n], x[k], s);
+    return s;
+}
+
+// The kb x kb diagonal block at (k0, k0) into D (FP64); a full block with all of a thread's
+// loads in flight before its LDS stores.
+template <typename T, int SB>
+__device__ __forceinline__ void stage_block(double (*D)[SB + 1], const T* A, int n, int k0, int kb) {
+    const int tid = threadIdx.x;
+    if (kb == SB) {
+        T v[SB * SB / BT];
+#pragma unroll
+        for (int u = 0; u < SB * SB / BT; ++u) {
+            const int idx = tid + u * BT, r = idx % SB, c = idx / SB;
+            v[u] = A[(int64_t)(k0 + c) * n + k0 + r];
+        }
+#pragma unroll
+        for (int u = 0; u < SB * SB / BT; ++u) {
+            const int idx = tid + u * BT;
+            D[idx % SB][idx / SB] = (double)v[u];
+        }
+    } else {
+        for (int idx = tid; idx < kb * kb; idx += BT) {
+            const int r = idx % kb, c = idx / kb;
+            D[r][c] = (double)A[(int64_t)(k0 + c) * n + k0 + r];
+        }
+    }
+}
+
 // sh.xs := U^{-1} L^{-1} sh.xs (sh.xs already holds P b), arithmetic in FP64.
 template <typename T>
 __device__ __forceinline__ void solve_sys(BatchShared<T>& sh, const T* A, int n) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int SB = BatchCfg<T>::SB;
+    static_assert((SB * SB) % BT == 0, "a full diagonal block is whole loads per thread");
     double (*D)[SB + 1] = sh.dblk;
     // forward: unit lower L
     for (int k0 = 0; k0 < n; k0 += SB) {
         const int kb = min(SB, n - k0);
-        for (int idx = tid; idx < kb * kb; idx += BT) {
-            const int r = idx % kb, c = idx / kb;
-            D[r][c] = (double)A[(int64_t)(k0 + c) * n + k0 + r];
-        }
+        stage_block<T, SB>(D, A, n, k0, kb);
         __syncthreads();
         if (wave == 0) {
             double xv = lane < kb ? sh.xs[k0 + lane] : 0.0;
@@ -456,20 +566,13 @@ __device__ __forceinline__ void solve_sys(BatchShared<T>& sh, const T* A, int n)
             if (lane < kb) sh.xs[k0 + lane] = xv;
         }
         __syncthreads();
-        for (int i = k0 + kb + tid; i < n; i += BT) {
-            double s = sh.xs[i];
-            for (int k = k0; k < k0 + kb; ++k) s = fma(-(double)A[(int64_t)k * n + i], sh.xs[k], s);
-            sh.xs[i] = s;
-        }
+        for (int i = k0 + kb + tid; i < n; i += BT) sh.xs[i] = row_dot_sub(sh.xs[i], A + i, n, sh.xs, k0, k0 + kb);
         __syncthreads();
     }
     // backward: U
     for (int k1 = n; k1 > 0; k1 -= SB) {
         const int k0 = max(0, k1 - SB), kb = k1 - k0;
-        for (int idx = tid; idx < kb * kb; idx += BT) {
-            const int r = idx % kb, c = idx / kb;
-            D[r][c] = (double)A[(int64_t)(k0 + c) * n + k0 + r];
-        }
+        stage_block<T, SB>(D, A, n, k0, kb);
         __syncthreads();
         if (wave == 0) {
             double xv = lane < kb ? sh.xs[k0 + lane] : 0.0;
@@ -481,11 +584,7 @@ __device__ __forceinline__ void solve_sys(BatchShared<T>& sh, const T* A, int n)
             if (lane < kb) sh.xs[k0 + lane] = xv;
         }
         __syncthreads();
-        for (int i = tid; i < k0; i += BT) {
-            double s = sh.xs[i];
-            for (int k = k0; k < k1; ++k) s = fma(-(double)A[(int64_t)k * n + i], sh.xs[k], s);
-            sh.xs[i] = s;
-        }
+        for (int i = tid; i < k0; i += BT) sh.xs[i] = row_dot_sub(sh.xs[i], A + i, n, sh.xs, k0, k1);
         __syncthreads();
     }
 }
@@ -512,7 +611,18 @@ __global__ __launch_bounds__(BT, BatchCfg<T>::WG * BT / 256) void k_batch_lu(Bat
         if (A64) {  // FP32 path: working copy of the FP64 matrix, by rows, with the row sums
             for (int i = tid; i < n; i += BT) {
                 double rs = 0.0;
-                for (int j = 0; j < n; ++j) {
+                int j = 0;
+                for (; j + 16 <= n; j += 16) {
+                    double v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) v[u] = A64[(j + u) * n + i];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        A[(j + u) * n + i] = (T)v[u];
+                        rs += fabs(v[u]);
+                    }
+                }
+                for (; j < n; ++j) {
                     const double v = A64[j * n + i];
                     A[j * n + i] = (T)v;
                     rs += fabs(v);
@@ -524,7 +634,8 @@ __global__ __launch_bounds__(BT, BatchCfg<T>::WG * BT / 256) void k_batch_lu(Bat
         __syncthreads();
         const int cfg = a.cfg ? a.cfg[sys] : default_cfg(n);
         if (cfg == 0) factor_sys<T, 8>(sh, A, n, vo, a.ipiv, info);
-        else factor_sys<T, 16>(sh, A, n, vo, a.ipiv, info);
+        else if (cfg == 1) factor_sys<T, 16>(sh, A, n, vo, a.ipiv, info);
+        else factor_sys<T, PMAX>(sh, A, n, vo, a.ipiv, info);
         // solve: x = U^{-1} L^{-1} P b
         for (int i = tid; i < n; i += BT) sh.xs[i] = a.b[vo + sh.perm[i]];
         __syncthreads();
@@ -542,8 +653,7 @@ __global__ __launch_bounds__(BT, BatchCfg<T>::WG * BT / 256) void k_batch_lu(Bat
             while (true) {
                 double rn = 0.0, xn = 0.0;
                 for (int i = tid; i < n; i += BT) {
-                    double s = a.b[vo + i];
-                    for (int j = 0; j < n; ++j) s = fma(-A64[(int64_t)j * n + i], sh.xr[j], s);
+                    const double s = row_dot_sub(a.b[vo + i], A64 + i, n, sh.xr, 0, n);
                     rr[i] = s;
                     rn = fmax(rn, fabs(s));
                     xn = fmax(xn, fabs(sh.xr[i]));
@@ -665,15 +775,15 @@ int batch_run(lsx_batch* B, const double* dA_src, const double* db, double* dx, 
 static int tiles_to_cfg(lsx_ctx* ctx, const std::vector<int32_t>& hn, int nbins, const int32_t* upper,
                         const int32_t* bw, std::vector<uint8_t>& cfg) {
     for (int k = 0; k < nbins; ++k)
-        if (bw[k] != 8 && bw[k] != 16)
-            return fail(ctx, LSX_EINVAL, "lsx_batch_set_tiles: panel width %d is not 8 or 16", bw[k]);
+        if (bw[k] != 8 && bw[k] != 16 && bw[k] != PMAX)
+            return fail(ctx, LSX_EINVAL, "lsx_batch_set_tiles: panel width %d is not 8, 16 or 32", bw[k]);
     cfg.resize(hn.size());
     for (size_t i = 0; i < hn.size(); ++i) {
         const int n = hn[i];
         int c = default_cfg(n);
         for (int k = 0; k < nbins; ++k) {
             if (n <= upper[k]) {
-                c = bw[k] == 8 ? 0 : 1;
+                c = bw[k] == 8 ? 0 : (bw[k] == 16 ? 1 : 2);
                 break;
             }
         }

@@ -117,15 +117,16 @@ def test_fp32_refined_blocks(lsx):
         off += n
 
 
-def test_fp32_refined_pivoting_blocks(lsx):
+@pytest.mark.parametrize("bw", [None, 8, 16, 32])
+def test_fp32_refined_pivoting_blocks(lsx, bw):
     # FP32 factors of systems that pivot in almost every column (iid N(0,1) blocks), sizes on both
-    # sides of 256 rows and of the panel widths: the interchanges applied in memory must leave
-    # factors that refine every block to the target
+    # sides of 256 rows and of the panel widths, with the tile table and with each panel width:
+    # the interchanges applied in memory must leave factors that refine every block to the target
     rng = np.random.default_rng(55)
-    sizes = list(rng.integers(1, 513, size=60)) + [1, 7, 8, 9, 15, 16, 17, 255, 256, 257, 511, 512]
+    sizes = list(rng.integers(1, 513, size=60)) + [1, 7, 8, 9, 15, 16, 17, 31, 32, 33, 255, 256, 257, 511, 512]
     blocks = [rng.standard_normal((int(s), int(s))) for s in sizes]
     b = rng.standard_normal(int(sum(sizes)))
-    alg = lsx.MI355XBatchedLUFactorization(precision="fp32", tol=1e-14)
+    alg = lsx.MI355XBatchedLUFactorization(precision="fp32", tol=1e-14, tiles="auto" if bw is None else [(512, bw)])
     sol = lsx.solve(lsx.LinearProblem(lsx.BlockDiagonal(blocks), b), alg)
     assert sol.retcode == lsx.ReturnCode.Success
     assert np.all(alg.last_info == 0) and np.all(alg.last_iters <= 10)
@@ -148,11 +149,11 @@ def test_fp32_singular_block(lsx):
     assert alg.last_info[0] == 0 and alg.last_info[1] == 0 and alg.last_info[2] > 0
 
 
-@pytest.mark.parametrize("bw", [8, 16])
+@pytest.mark.parametrize("bw", [8, 16, 32])
 def test_tile_widths_keep_lapack_pivots(lsx, bw):
     # every panel width of the tile table gives LAPACK's pivots and a backward error <= 1e-13
     rng = np.random.default_rng(7)
-    sizes = list(rng.integers(1, 513, size=60)) + [7, 8, 9, 15, 16, 17, 512]
+    sizes = list(rng.integers(1, 513, size=60)) + [7, 8, 9, 15, 16, 17, 31, 32, 33, 63, 64, 65, 512]
     blocks = [rng.standard_normal((int(s), int(s))) for s in sizes]
     b = rng.standard_normal(int(sum(sizes)))
     alg = lsx.MI355XBatchedLUFactorization(tiles=[(512, bw)])
