@@ -32,12 +32,13 @@ template <> struct GemmMma<float> {
 // (B^T A^T) so each lane's results run down a column of C (128-byte stores).
 constexpr int GBK = 16;
 constexpr int GGM = 8;  // tile-rows per group of the tile order (L2 reuse of A and B blocks)
-template <typename T, int MODE, int BM, int BN>
-__global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t lda, const T* Bg, int64_t ldb,
-                                                 T* Cg, int64_t ldc, int M, int N, int K) {
+template <typename T, int MODE, int BM, int BN, int NT = 256>
+__global__ __launch_bounds__(NT, 2 * NT / 256) void k_gemm(const T* __restrict Ag, int64_t lda, const T* Bg, int64_t ldb,
+                                                           T* Cg, int64_t ldc, int M, int N, int K) {
     typedef GemmMma<T> G;
-    constexpr int WI = BN / 32, WJ = BM / 32;       // 16 x 16 blocks per wave along C columns / rows
-    constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;  // staged elements per thread
+    constexpr int WNW = NT / 128;                    // waves along C columns (two along rows)
+    constexpr int WI = BN / (16 * WNW), WJ = BM / 32;  // 16 x 16 blocks per wave along C columns / rows
+    constexpr int LA = BM * GBK / NT, LB = BN * GBK / NT;  // staged elements per thread
     const int TM = (M + BM - 1) / BM, TN = (N + BN - 1) / BN, NTL = TM * TN;
     // XCD-aware: workgroup b runs on XCD b mod 8, which takes a contiguous range of the tile
     // order; in that order groups of GGM tile-rows are swept column by column, so the tiles
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t
     __shared__ T As[2][GBK][BM + 1];
     __shared__ T Bs[2][GBK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave & 1, wn = wave >> 1;  // wave (wm, wn): rows wm BM/2 .., columns wn BN/WNW ..
     typename G::acc_t acc[WI][WJ];
 #pragma unroll
     for (int i = 0; i < WI; ++i)
@@ -63,20 +64,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t
 
     // lane holds C(row = rbase + (lane & 15), col = cbase + G::ccol(lane, q)) of each 16 x 16 block
     auto crow = [&](int j) { return row0 + wm * (BM / 2) + j * 16 + (lane & 15); };
-    auto ccol = [&](int i, int q) { return col0 + wn * (BN / 2) + i * 16 + G::ccol(lane, q); };
+    auto ccol = [&](int i, int q) { return col0 + wn * (BN / WNW) + i * 16 + G::ccol(lane, q); };
 
     T ra[LA], rb[LB];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int it = 0; it < LA; ++it) {
-            const int idx = it * 256 + tid;
+            const int idx = it * NT + tid;
             const int r = idx % BM, k = idx / BM;
             const int gr = row0 + r, gk = k0 + k;
             ra[it] = (gr < M && gk < K) ? Ag[(int64_t)gk * lda + gr] : T(0);
         }
 #pragma unroll
         for (int it = 0; it < LB; ++it) {
-            const int idx = it * 256 + tid;
+            const int idx = it * NT + tid;
             const int kb = idx & (GBK - 1), c = idx >> 4;
             const int gc = col0 + c, gk2 = k0 + kb;
             rb[it] = (gc < N && gk2 < K) ? Bg[(int64_t)gc * ldb + gk2] : T(0);
@@ -85,12 +86,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int it = 0; it < LA; ++it) {
-            const int idx = it * 256 + tid;
+            const int idx = it * NT + tid;
             As[buf][idx / BM][idx % BM] = ra[it];
         }
 #pragma unroll
         for (int it = 0; it < LB; ++it) {
-            const int idx = it * 256 + tid;
+            const int idx = it * NT + tid;
             Bs[buf][idx & (GBK - 1)][idx >> 4] = rb[it];
         }
     };
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t
             const int kr = k4 * 4 + (lane >> 4);
             T x[WI], y[WJ];
 #pragma unroll
-            for (int i = 0; i < WI; ++i) x[i] = Bs[buf][kr][wn * (BN / 2) + i * 16 + (lane & 15)];
+            for (int i = 0; i < WI; ++i) x[i] = Bs[buf][kr][wn * (BN / WNW) + i * 16 + (lane & 15)];
 #pragma unroll
             for (int j = 0; j < WJ; ++j) y[j] = As[buf][kr][wm * (BM / 2) + j * 16 + (lane & 15)];
 #pragma unroll
@@ -158,13 +159,18 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const T* __restrict Ag, int64_t
     }
 }
 
-template <typename T, int MODE, int BM, int BN>
+template <typename T, int MODE, int BM, int BN, int NT = 256>
 static void launch_gemm(hipStream_t st, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t M,
                         int64_t N, int64_t K) {
     const int64_t T_ = cdiv(M, BM) * cdiv(N, BN);
-    hipLaunchKernelGGL((k_gemm<T, MODE, BM, BN>), dim3((unsigned)(8 * cdiv(T_, 8))), dim3(256), 0, st, A, lda, B, ldb,
-                       C, ldc, (int)M, (int)N, (int)K);
+    hipLaunchKernelGGL((k_gemm<T, MODE, BM, BN, NT>), dim3((unsigned)(8 * cdiv(T_, 8))), dim3(NT), 0, st, A, lda, B,
+                       ldb, C, ldc, (int)M, (int)N, (int)K);
 }
+
+// Threads of the big-tile trailing update: FP64 runs eight waves of 64 x 32 per 128 x 128 tile
+// (two workgroups, sixteen waves per CU), FP32 four waves of 64 x 64.
+template <typename T> struct GemmBigThreads { static constexpr int NT = 256; };
+template <> struct GemmBigThreads<double> { static constexpr int NT = 512; };
 
 // C -= A B (mode 0) or C = A B in place on B (mode 1, M <= 64) on stream st: 128 x 128 tiles
 // when they give every CU two workgroups, 64 x 64 otherwise.  Returns whether big tiles ran.
@@ -173,7 +179,7 @@ static bool gemm_dispatch(hipStream_t st, int num_cus, const T* A, int64_t lda, 
                           int64_t ldc, int64_t M, int64_t N, int64_t K, int mode) {
     const bool big = mode == 0 && cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)num_cus;
     if (mode == 0) {
-        if (big) launch_gemm<T, 0, 128, 128>(st, A, lda, B, ldb, C, ldc, M, N, K);
+        if (big) launch_gemm<T, 0, 128, 128, GemmBigThreads<T>::NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
         else launch_gemm<T, 0, 64, 64>(st, A, lda, B, ldb, C, ldc, M, N, K);
     } else {
         if (M <= 64) launch_gemm<T, 1, 64, 64>(st, A, lda, B, ldb, C, ldc, M, N, K);
