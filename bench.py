@@ -838,8 +838,8 @@ def main():
                                   ms_per_step=prof.ms[i] / psteps)
     g = _abi.PROFILE_CLASSES.index("gemm")
     gemm_tflops = prof.flops[g] / (prof.ms[g] / 1e3) / 1e12 if prof.ms[g] > 0 else 0.0
-    kname = "k_gemm<double, 0, 128, 128>"
-    roofline = dict(bound="mfma", kernel=f"{kname} (trailing update A22 -= L21*U12, 128x128 tiles, K=128)",
+    kname = "k_gemm<double, 0, 128, 128, 512>"
+    roofline = dict(bound="mfma", kernel=f"{kname} (trailing update A22 -= L21*U12, 128x128 tiles on eight waves, K=128)",
                     achieved=round(gemm_tflops, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=round(gemm_tflops / FP64_PEAK_TFLOPS, 4),
                     flops_per_launch=prof.flops[g] / max(prof.launches[g], 1),
