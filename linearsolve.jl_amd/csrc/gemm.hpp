@@ -167,10 +167,10 @@ static void launch_gemm(hipStream_t st, const T* A, int64_t lda, const T* B, int
                        ldb, C, ldc, (int)M, (int)N, (int)K);
 }
 
-// Threads per tile: FP64 runs eight waves (64 x 32 each on a 128 x 128 tile, 32 x 16 on a
-// 64 x 64 tile), FP32 four.
-template <typename T> struct GemmThreads { static constexpr int NT = 256; };
-template <> struct GemmThreads<double> { static constexpr int NT = 512; };
+// Threads per tile: eight waves (64 x 32 each on a 128 x 128 tile, 32 x 16 on a 64 x 64 tile),
+// two workgroups per CU.  Four waves of 64 x 64 per 128 x 128 tile need 255 registers, so a CU
+// held eight waves; these take 128, so it holds sixteen (FP64 trailing update 812 -> 722 us).
+constexpr int GEMM_NT = 512;
 
 // C -= A B (mode 0) or C = A B in place on B (mode 1, M <= 64) on stream st: 128 x 128 tiles
 // when they give every CU two workgroups, 64 x 64 otherwise.  Returns whether big tiles ran.
@@ -179,10 +179,10 @@ static bool gemm_dispatch(hipStream_t st, int num_cus, const T* A, int64_t lda, 
                           int64_t ldc, int64_t M, int64_t N, int64_t K, int mode) {
     const bool big = mode == 0 && cdiv(M, 128) * cdiv(N, 128) >= 2 * (int64_t)num_cus;
     if (mode == 0) {
-        if (big) launch_gemm<T, 0, 128, 128, GemmThreads<T>::NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
-        else launch_gemm<T, 0, 64, 64, GemmThreads<T>::NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
+        if (big) launch_gemm<T, 0, 128, 128, GEMM_NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
+        else launch_gemm<T, 0, 64, 64, GEMM_NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
     } else {
-        if (M <= 64) launch_gemm<T, 1, 64, 64, GemmThreads<T>::NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
+        if (M <= 64) launch_gemm<T, 1, 64, 64, GEMM_NT>(st, A, lda, B, ldb, C, ldc, M, N, K);
         else launch_gemm<T, 1, 128, 128>(st, A, lda, B, ldb, C, ldc, M, N, K);
     }
     return big;
