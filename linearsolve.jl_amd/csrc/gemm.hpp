@@ -27,8 +27,8 @@ template <> struct GemmMma<float> {
 //   MODE 0: C -= A * B      MODE 1: C = A * B (in place on B when M <= BM: a workgroup reads all
 //                           K rows of its B columns before it writes C)
 // A: M x K (lda), B: K x N (ldb), C: M x N (ldc), column-major.  BM x BN tile per workgroup
-// (128 x 128, or 64 x 64 when the big tiles cannot fill the GPU), four waves of BM/2 x BN/2,
-// K staged 16 at a time with double buffering.  The MFMA computes the transposed tile
+// (128 x 128, or 64 x 64 when the big tiles cannot fill the GPU), NT / 64 waves of
+// BM/2 x BN/(NT/128) (two along the rows), K staged 16 at a time with double buffering.  The MFMA computes the transposed tile
 // (B^T A^T) so each lane's results run down a column of C (128-byte stores).
 constexpr int GBK = 16;
 constexpr int GGM = 8;  // tile-rows per group of the tile order (L2 reuse of A and B blocks)
