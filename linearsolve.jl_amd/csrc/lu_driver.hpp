@@ -103,9 +103,6 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
             const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
             hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(256), 0, lu->cur, F, ld, n, b0, DL, DU);
             lu->prof.end(lu->cur, pi);
-            if (defer)
-                hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
-                                   lu->pinv + (jo / NBO) * n, n);
             return LSX_OK;
         }
         for (int64_t ji = jo; ji < jo + jbo; ji += PW) {
@@ -132,10 +129,14 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
         const int pi = lu->prof.begin(lu->cur, PK_OTHER, 0.0, 0.0);
         hipLaunchKernelGGL(k_diag_inv<T>, dim3((unsigned)nbk, 2), dim3(256), 0, lu->cur, F, ld, n, b0, DL, DU);
         lu->prof.end(lu->cur, pi);
+        return LSX_OK;
+    };
+    // the inverse of the row map after block jo, for the deferred L-column gather; launched after
+    // the block's ev_panel so that the update stream does not wait for it
+    auto inv_perm = [&](int64_t jo) {
         if (defer)
             hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, lu->cur, lu->perm,
                                lu->pinv + (jo / NBO) * n, n);
-        return LSX_OK;
     };
     // interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 of block [jo, jo + jbo) on columns
     // [c0, c1); wait_inv: the block's diagonal-block inverses may still be in the making on the
@@ -161,6 +162,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
     int rc_ = panel_block(0, (int)std::min<int64_t>(NBO, n), 0);
     if (rc_) return rc_;
     LSX_HIP(ctx, hipEventRecord(lu->ev_panel, s0));
+    inv_perm(0);
     int par = 0;
     for (int64_t jo = 0; jo < n; jo += NBO, par ^= 1) {
         const int jbo = (int)std::min<int64_t>(NBO, n - jo);
@@ -176,6 +178,7 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
         lu->cur = s1;
         if ((rc_ = panel_block(c1, jbn, par ^ 1))) return rc_;
         LSX_HIP(ctx, hipEventRecord(lu->ev_panel, s1));
+        inv_perm(c1);
         lu->cur = s0;
         if ((rc_ = update(jo, jbo, par, c1 + jbn, n, false))) return rc_;
         // the next block's interchange list (ev_perm) is all the next iteration's first step
@@ -184,6 +187,10 @@ int getrf_blocked(lsx_lu* lu, T* F, int64_t ld, T* DL, T* DU, GU gemm_upd, GT ge
     }
     lu->cur = s0;
     LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_panel, 0));  // the last block's inverses and row map
+    if (defer && s1 != s0) {  // and every block's inverse row map (launched on s1 after its ev_panel)
+        LSX_HIP(ctx, hipEventRecord(lu->ev_next, s1));
+        LSX_HIP(ctx, hipStreamWaitEvent(s0, lu->ev_next, 0));
+    }
     if (defer && n > NBO) {
         const int64_t ncols = (cdiv(n, NBO) - 1) * NBO;
         const size_t lds = sizeof(T) * (size_t)(n - NBO);
