@@ -488,8 +488,10 @@ extern "C" int lsx_lu_refine(lsx_lu* lu, int64_t nrhs, const double* B, int64_t 
                     use_gmres = 1;
                     mode_used = 1;
                     stall = 0;
-                } else if (stall >= 3 && use_gmres && lu->dtype == LSX_BF16 && !escalated) {
-                    // GMRES-IR stalls too (u_bf16 * cond(A) >= 1): refactor in FP32, keep x.
+                } else if (stall >= 1 && use_gmres && lu->dtype == LSX_BF16 && !escalated) {
+                    // GMRES-IR stalls too (u_bf16 * cond(A) >= 1): refactor in FP32, keep x.  One
+                    // GMRES-IR step that does not halve the backward error decides it: an FP32
+                    // factorization costs about as much as 60 inner GMRES iterations.
                     // The refactorization clears the wait error word, so a wait that gave up in
                     // the solves so far is reported first.
                     LSX_HIP(ctx, hipMemcpyAsync(lu->hinfo + 1, lu->dinfo + 1, sizeof(int64_t),
