@@ -11,16 +11,18 @@
 //
 // Scheduling: systems are sorted by decreasing size on the host (longest first); a persistent
 // grid of workgroups takes them from an atomic counter, one workgroup per system.
-// Per system (BT threads): right-looking blocked LU with panels of BW columns held in LDS.  A
-// panel's row interchanges are applied in memory to the columns right of it as U12 is formed
-// (one thread per column moves the at most 2 BW rows involved), so the trailing update
-// A22 -= L21 U12 reads and writes contiguous rows; it runs on MFMA (FP64 16x16x4 with 64 x 32
-// blocks per wave, L21 from LDS; FP32 16x16x4 with 32 x 32 blocks, L21 from L2) and U12 from L2.  The L columns take the
-// interchanges of later panels in one gather after the last panel (like dlaswp on the left, but
-// once).  The panel width BW (8 or 16) is chosen per system from a size-binned tile
-// table (set by the autotuner through lsx_batch_set_tiles).  Triangular solves go by blocks of 64
-// rows: the diagonal block is staged in LDS and solved by one wave with shuffles, the rows
-// beyond it are updated by all threads.
+// Per system (BT threads): right-looking blocked LU with panels of BW columns factored in
+// registers (pivot search on packed keys by DPP).  A panel's row interchanges are applied in
+// memory to the columns right of it as U12 is formed (one thread per column moves the at most
+// 2 BW rows involved), so the trailing update A22 -= L21 U12 reads and writes contiguous rows;
+// it runs on MFMA: FP64 panels of 8 or 16 columns hold L21 in LDS and deal 64 x 32 tiles to the
+// waves in turn; 32-column panels (and every FP32 panel) keep a 32-row block of L21 in each
+// wave's registers and sweep the 32 x 32 tiles of its rows, U12 read from L2.  The L columns
+// take the interchanges of later panels in one gather after the last panel (like dlaswp on the
+// left, but once).  The panel width BW (8, 16 or 32) is chosen per system from a size-binned
+// tile table (set by the autotuner through lsx_batch_set_tiles).  Triangular solves go by
+// blocks of 64 (FP32: 32) rows: the diagonal block is staged in LDS and solved by one wave with
+// shuffles, the rows beyond it are updated by all threads, sixteen loads in flight per row.
 // The launch sequence of a handle (counter reset + persistent kernel) is captured once in a HIP
 // graph and replayed by every factor_solve call (LSX_BATCH_NO_GRAPH=1 launches it directly).
 #include <algorithm>
