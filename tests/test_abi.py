@@ -112,3 +112,22 @@ def test_default_algorithm_routes():
         lsx.default_algorithm(np.ones((4, 3)), b)
     with pytest.raises(ValueError):
         lsx.OperatorAssumptions(condition="Unknown")
+
+
+def test_tile_tables_per_factor_type(tmp_path):
+    # tile_table.json holds one size-binned panel-width table per factor type; a file without an
+    # FP32 table serves FP32 batches from the FP64 one, and no file means the library's default
+    import json
+    lsx = load_package()
+    p = tmp_path / "tiles.json"
+    p.write_text(json.dumps({"bins": [[64, 8], [512, 32]], "bins_fp32": [[128, 16], [512, 32]]}))
+    assert lsx.load_tile_table(str(p), dtype=lsx.LSX_F64) == [(64, 8), (512, 32)]
+    assert lsx.load_tile_table(str(p), dtype=lsx.LSX_F32) == [(128, 16), (512, 32)]
+    p.write_text(json.dumps({"bins": [[512, 16]]}))
+    assert lsx.load_tile_table(str(p), dtype=lsx.LSX_F32) == [(512, 16)]
+    assert lsx.load_tile_table(str(tmp_path / "none.json")) is None
+    # the committed table covers every size a batch accepts (n <= 512) with widths the library takes
+    for dt in (lsx.LSX_F64, lsx.LSX_F32):
+        t = lsx.load_tile_table(dtype=dt)
+        if t is not None:
+            assert max(u for u, _ in t) >= 512 and all(w in (8, 16, 32) for _, w in t)
