@@ -126,7 +126,6 @@ __device__ __forceinline__ int64_t uniform64(int64_t x) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-
 // MFMA of the factor type: the transposed 16 x 16 tile D = X Y (X(i, k) = -U12(k, col i),
 // Y(k, j) = L21(row j, k)); lane l / slot q holds C(row rb + (l & 15), col cb + ccol(l, q)).
 template <typename T> struct BMma;
