@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-DEFAULT_BINS = (32, 64, 128, 256, 384, 512)
+DEFAULT_BINS = (32, 64, 96, 128, 192, 256, 320, 384, 448, 512)
 WIDTHS = (8, 16, 32)
 DTYPES = (("bins", "LSX_F64"), ("bins_fp32", "LSX_F32"))
 
