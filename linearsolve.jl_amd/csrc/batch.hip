@@ -422,35 +422,74 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
                 // A22 -= L21 U12 on MFMA by row blocks of 16 R2 rows: a wave loads its row block
                 // of L21 into registers once and sweeps the column blocks (16 C2 columns) of a
                 // segment of A22's columns; with fewer row blocks than waves, the columns are cut
-                // into segments so that every wave has work.  U12 is read per tile (out of L2).
+                // into segments so that every wave has work.  FP64: U12 is staged in LDS (the
+                // held-L21 area, free for 32-column panels), UCH columns at a time, so the waves
+                // do not read it again from L2 or HBM for every row block; FP32 reads it per tile
+                // (out of L2).
                 constexpr int R2 = BatchCfg<T>::RB2, C2 = BatchCfg<T>::CB2;
-                const int tbr = (mr + 16 * R2 - 1) / (16 * R2), tbc = (mr + 16 * C2 - 1) / (16 * C2);
-                const int nseg = min(tbc, (NW + tbr - 1) / tbr);
-                const int segw = (tbc + nseg - 1) / nseg;
-                for (int u = wave; u < tbr * nseg; u += NW) {
-                    const int rb = (u % tbr) * (16 * R2), sg = u / tbr;
-                    T yl[R2][KS];
+                constexpr bool STG = BatchCfg<T>::HOLD;
+                constexpr int UCH = STG ? (BNMAX * 17) / (BW + 1) / (16 * C2) * (16 * C2) : BNMAX;
+                static_assert(!STG || UCH >= 16 * C2, "a staged U12 chunk holds a column block");
+                const int tbr = (mr + 16 * R2 - 1) / (16 * R2);
+                for (int cc0 = 0; cc0 < mr; cc0 += UCH) {
+                    const int ncc = min(UCH, mr - cc0);
+                    if constexpr (STG) {
+                        double* Us = &sh.l21[0];  // Us[c (BW + 1) + k] = -U12(k, column cc0 + c)
+                        __syncthreads();          // the previous chunk's tiles are done with Us
+                        for (int base = tid; base < ncc * BW; base += 8 * BT) {
+                            T t8[8];
 #pragma unroll
-                    for (int a2 = 0; a2 < R2; ++a2) {
-                        const int lr = rb + 16 * a2 + (lane & 15);
+                            for (int u8 = 0; u8 < 8; ++u8) {
+                                const int idx = base + u8 * BT, c = idx / BW, k = idx - c * BW;
+                                t8[u8] = idx < ncc * BW ? U12[(cc0 + c) * n + k] : T(0);
+                            }
 #pragma unroll
-                        for (int k4 = 0; k4 < KS; ++k4) yl[a2][k4] = lr < mr ? L21[(k4 * 4 + (lane >> 4)) * n + lr] : T(0);
+                            for (int u8 = 0; u8 < 8; ++u8) {
+                                const int idx = base + u8 * BT, c = idx / BW, k = idx - c * BW;
+                                if (idx < ncc * BW) Us[c * (BW + 1) + k] = -(double)t8[u8];
+                            }
+                        }
+                        __syncthreads();
                     }
-                    const int cbs = sg * segw, cbe = min(tbc, (sg + 1) * segw);
-                    for (int cbi = cbs; cbi < cbe; ++cbi) {
-                        const int cb = cbi * (16 * C2);
-                        acc_t acc[R2][C2];
-                        T xv[C2][KS];
-                        load_c(acc, rb, cb);
-                        load_u(xv, cb);
+                    const int tbc = (ncc + 16 * C2 - 1) / (16 * C2);
+                    const int nseg = min(tbc, (NW + tbr - 1) / tbr);
+                    const int segw = (tbc + nseg - 1) / nseg;
+                    for (int u = wave; u < tbr * nseg; u += NW) {
+                        const int rb = (u % tbr) * (16 * R2), sg = u / tbr;
+                        T yl[R2][KS];
 #pragma unroll
-                        for (int k4 = 0; k4 < KS; ++k4)
+                        for (int a2 = 0; a2 < R2; ++a2) {
+                            const int lr = rb + 16 * a2 + (lane & 15);
 #pragma unroll
-                            for (int a2 = 0; a2 < R2; ++a2)
+                            for (int k4 = 0; k4 < KS; ++k4) yl[a2][k4] = lr < mr ? L21[(k4 * 4 + (lane >> 4)) * n + lr] : T(0);
+                        }
+                        const int cbs = sg * segw, cbe = min(tbc, (sg + 1) * segw);
+                        for (int cbi = cbs; cbi < cbe; ++cbi) {
+                            const int cb = cc0 + cbi * (16 * C2);
+                            acc_t acc[R2][C2];
+                            T xv[C2][KS];
+                            load_c(acc, rb, cb);
+                            if constexpr (STG) {
+                                const double* Us = &sh.l21[0];
 #pragma unroll
-                                for (int b2 = 0; b2 < C2; ++b2)
-                                    acc[a2][b2] = BMma<T>::mma(xv[b2][k4], yl[a2][k4], acc[a2][b2]);
-                        store_c(acc, rb, cb);
+                                for (int b2 = 0; b2 < C2; ++b2) {
+                                    const int lc = cb + 16 * b2 + (lane & 15);
+#pragma unroll
+                                    for (int k4 = 0; k4 < KS; ++k4)
+                                        xv[b2][k4] = (lc < mr) ? (T)Us[(lc - cc0) * (BW + 1) + k4 * 4 + (lane >> 4)] : T(0);
+                                }
+                            } else {
+                                load_u(xv, cb);
+                            }
+#pragma unroll
+                            for (int k4 = 0; k4 < KS; ++k4)
+#pragma unroll
+                                for (int a2 = 0; a2 < R2; ++a2)
+#pragma unroll
+                                    for (int b2 = 0; b2 < C2; ++b2)
+                                        acc[a2][b2] = BMma<T>::mma(xv[b2][k4], yl[a2][k4], acc[a2][b2]);
+                            store_c(acc, rb, cb);
+                        }
                     }
                 }
             }
