@@ -50,9 +50,10 @@ static_assert(RPT <= 2, "a pivot row is held by the first or the last of a threa
 // tiles, whether L21 and the pivot row are held (L21 in LDS, the pivot row in registers) or read
 // where they are used, and the row block SB of the triangular solves.
 // FP64: two systems per CU (up to 256 registers, 80 KB of LDS each) with L21 in LDS and 64-row
-// solve blocks.  FP32: four systems per CU (128 registers, 21 KB of LDS each), L21 read back
-// from the factor in L2 and 32-row solve blocks; the extra systems hide the latency of its
-// refinement passes.  (FP64 at four systems per CU spills registers and runs slower.)
+// solve blocks.  FP32: three systems per CU (168 registers, 36 KB of LDS each: 17 KB of them
+// stage U12 of 32-column panels), L21 read back from the factor in L2 and 32-row solve blocks;
+// the extra systems hide the latency of its refinement passes.  (FP64 at four systems per CU
+// spills registers and runs slower; FP32 at four spills 129 registers once U12 is staged.)
 template <typename T> struct BatchCfg;
 // Panels of 32 columns never hold L21 (it would take 136 KB of LDS); their trailing update
 // (and every FP32 one) keeps a row block of L21 in registers and sweeps the column blocks of
@@ -64,10 +65,11 @@ template <> struct BatchCfg<double> {
     static constexpr bool HOLD = true;
 };
 template <> struct BatchCfg<float> {
-    static constexpr int WG = 4, RB = 2, CB = 2, RB2 = 2, CB2 = 2, SB = 32;
+    static constexpr int WG = 3, RB = 2, CB = 2, RB2 = 2, CB2 = 2, SB = 32;
     static constexpr bool HOLD = false;
 };
 constexpr int PMAX = 32;  // widest panel
+constexpr int UCH_F = 128;  // FP32: columns of U12 staged in LDS at a time
 // Whether a panel of width BW holds L21 in LDS and the pivot row in registers.
 template <typename T, int BW> struct PanelHold { static constexpr bool value = BatchCfg<T>::HOLD && BW <= 16; };
 
@@ -112,6 +114,9 @@ struct BatchShared {
     double red[2][BT / 64];
     int chg[PMAX];             // rows below the panel that its interchanges move
     int nchg;
+    // FP32, 32-column panels: staged U12 of the trailing update (UCH_F columns, padded rows);
+    // FP64 stages it in the held-L21 area instead
+    T ustg[BatchCfg<T>::HOLD ? 1 : UCH_F * (PMAX + 1)];
     double xs[BNMAX];          // solve vector
     union {
         double xr[BNMAX];      // FP32 path: the refined solution (after the factorization)
@@ -422,20 +427,25 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
                 // A22 -= L21 U12 on MFMA by row blocks of 16 R2 rows: a wave loads its row block
                 // of L21 into registers once and sweeps the column blocks (16 C2 columns) of a
                 // segment of A22's columns; with fewer row blocks than waves, the columns are cut
-                // into segments so that every wave has work.  FP64: U12 is staged in LDS (the
-                // held-L21 area, free for 32-column panels), UCH columns at a time, so the waves
-                // do not read it again from L2 or HBM for every row block; FP32 reads it per tile
-                // (out of L2).
+                // into segments so that every wave has work.  32-column panels stage U12 in LDS,
+                // UCH columns at a time (FP64: in the held-L21 area; FP32: in a buffer of its
+                // own), so the waves do not read it again from L2 or HBM for every row block;
+                // narrower FP32 panels read it per tile (out of L2).
                 constexpr int R2 = BatchCfg<T>::RB2, C2 = BatchCfg<T>::CB2;
-                constexpr bool STG = BatchCfg<T>::HOLD;
-                constexpr int UCH = STG ? (BNMAX * 17) / (BW + 1) / (16 * C2) * (16 * C2) : BNMAX;
-                static_assert(!STG || UCH >= 16 * C2, "a staged U12 chunk holds a column block");
+                constexpr bool HOLDT = BatchCfg<T>::HOLD;
+                constexpr bool STG = BW == PMAX;
+                constexpr int UCH = !STG ? BNMAX : (HOLDT ? (BNMAX * 17) / (BW + 1) / (16 * C2) * (16 * C2) : UCH_F);
+                static_assert(UCH >= 16 * C2 && UCH % (16 * C2) == 0, "a staged U12 chunk holds whole column blocks");
+                auto ubuf = [&]() -> T* {
+                    if constexpr (HOLDT) return reinterpret_cast<T*>(&sh.l21[0]);
+                    else return &sh.ustg[0];
+                };
                 const int tbr = (mr + 16 * R2 - 1) / (16 * R2);
                 for (int cc0 = 0; cc0 < mr; cc0 += UCH) {
                     const int ncc = min(UCH, mr - cc0);
                     if constexpr (STG) {
-                        double* Us = &sh.l21[0];  // Us[c (BW + 1) + k] = -U12(k, column cc0 + c)
-                        __syncthreads();          // the previous chunk's tiles are done with Us
+                        T* Us = ubuf();   // Us[c (BW + 1) + k] = -U12(k, column cc0 + c)
+                        __syncthreads();  // the previous chunk's tiles are done with Us
                         for (int base = tid; base < ncc * BW; base += 8 * BT) {
                             T t8[8];
 #pragma unroll
@@ -446,7 +456,7 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
 #pragma unroll
                             for (int u8 = 0; u8 < 8; ++u8) {
                                 const int idx = base + u8 * BT, c = idx / BW, k = idx - c * BW;
-                                if (idx < ncc * BW) Us[c * (BW + 1) + k] = -(double)t8[u8];
+                                if (idx < ncc * BW) Us[c * (BW + 1) + k] = -t8[u8];
                             }
                         }
                         __syncthreads();
@@ -470,13 +480,13 @@ __device__ __forceinline__ void factor_sys(BatchShared<T>& sh, T* A, int n, int6
                             T xv[C2][KS];
                             load_c(acc, rb, cb);
                             if constexpr (STG) {
-                                const double* Us = &sh.l21[0];
+                                const T* Us = ubuf();
 #pragma unroll
                                 for (int b2 = 0; b2 < C2; ++b2) {
                                     const int lc = cb + 16 * b2 + (lane & 15);
 #pragma unroll
                                     for (int k4 = 0; k4 < KS; ++k4)
-                                        xv[b2][k4] = (lc < mr) ? (T)Us[(lc - cc0) * (BW + 1) + k4 * 4 + (lane >> 4)] : T(0);
+                                        xv[b2][k4] = (lc < mr) ? Us[(lc - cc0) * (BW + 1) + k4 * 4 + (lane >> 4)] : T(0);
                                 }
                             } else {
                                 load_u(xv, cb);
