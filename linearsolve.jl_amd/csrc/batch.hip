@@ -73,8 +73,9 @@ constexpr int UCH_F = 128;  // FP32: columns of U12 staged in LDS at a time
 // Whether a panel of width BW holds L21 in LDS and the pivot row in registers.
 template <typename T, int BW> struct PanelHold { static constexpr bool value = BatchCfg<T>::HOLD && BW <= 16; };
 
-// Panel-width index (0: 8, 1: 16, 2: 32 columns) of a system without a tile table.
-__host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : 1; }
+// Panel-width index (0: 8, 1: 16, 2: 32 columns) of a system without a tile table (the widths
+// the autotuner picks on MI355X for both factor types; the serving lane's batches use these).
+__host__ __device__ inline int default_cfg(int n) { return n <= 64 ? 0 : (n <= 192 ? 1 : 2); }
 
 struct BatchArgs {
     int64_t count;
